@@ -1,0 +1,150 @@
+"""Benchmark: RHF Fock build (fused Rys ERI + J/K) on one MI355X.
+
+Workload (default): n-hexadecane C16H34, all-trans, 6-31G (148 shells, 212
+cartesian functions), Schwarz threshold 1e-12, FP64.  One step = one full
+two-electron Fock build G = J - K/2 from a density already resident in HBM
+(core-Hamiltonian guess density), i.e. every surviving canonical shell
+quartet's ERIs are recomputed and contracted.
+
+Prints one JSON line (metric, value = canonical surviving ERI quartets per
+second, ms_per_step = Fock-build time per iteration, roofline of the
+dominant kernel from HIP events, CPU baseline from the oracle on a bounded
+sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
+HBM_PEAK = 8000.0         # GB/s
+
+
+def core_guess_density(m):
+    from librint_amd import scf
+    S = scf.int1e(m, "ovlp")
+    H = scf.int1e(m, "kin") + scf.int1e(m, "nuc")
+    s, U = np.linalg.eigh(S)
+    X = U / np.sqrt(s)
+    e, C = np.linalg.eigh(X.T @ H @ X)
+    C = X @ C
+    nocc = m.nelectron // 2
+    return 2.0 * C[:, :nocc] @ C[:, :nocc].T
+
+
+def workload(name):
+    from librint_amd import mol
+    if name == "c16h34_631g":
+        return mol.c16h34_631g(), 1e-12
+    if name == "h2o_sto3g":
+        return mol.h2o_sto3g(), 0.0
+    if name.startswith("alkane"):
+        return mol.alkane_631g(int(name[6:])), 1e-12
+    raise SystemExit("unknown workload " + name)
+
+
+def cpu_baseline(m, P, seconds, seed=0):
+    """Oracle (plain-C restatement of integral2e_fock: full nbas^4 loop, no
+    symmetry, no screening, FP64, one core) on a random sample of ordered
+    shell quartets; returns the rate and the extrapolated Fock-build time."""
+    from oracle import refcpu
+    nb = m.nbas
+    rng = np.random.default_rng(seed)
+    total = 0
+    t_used = 0.0
+    batch = 2000
+    while t_used < seconds:
+        q = rng.integers(0, nb, size=(batch, 4)).astype(np.int32)
+        t0 = time.perf_counter()
+        refcpu.fock2e_sample(m, P, q)
+        t_used += time.perf_counter() - t0
+        total += batch
+    rate = total / t_used
+    return rate, float(nb) ** 4 / rate, total, t_used
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c16h34_631g")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the Fock build is sharded by quartet bins "
+              "across GPUs in the design, not in this build); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from librint_amd.context import FockContext
+
+    m, tau = workload(args.workload)
+    P = core_guess_density(m)
+    ctx = FockContext(m, schwarz_tol=tau)
+    dev = torch.device("cuda:0")
+    Pd = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
+    Gd = torch.empty_like(Pd)
+    for _ in range(args.warmup):
+        ctx.fock_device(Pd, Gd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.fock_device(Pd, Gd)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # device time of the J/K kernels (HIP events recorded by the library on
+    # its stream around the class launches), one more timed build each
+    ev_ms = []
+    for _ in range(min(args.steps, 10)):
+        ctx.fock(P)
+        ev_ms.append(ctx.stats()["kernel_ms"])
+    st = ctx.stats()
+    ms_per_step = wall / args.steps * 1e3
+    quartets = st["quartets"]
+    jk_ms = float(np.mean(ev_ms))
+    achieved = st["flops"] / (jk_ms * 1e-3) / 1e12
+    line = {
+        "metric": "RHF Fock-build s/iter + ERI quartets/s (FP64 fused J/K)",
+        "value": quartets / (ms_per_step * 1e-3),
+        "unit": "quartets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "fock_s_per_iter": ms_per_step * 1e-3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic geometry (all-trans zig-zag, ideal angles), core-Hamiltonian density",
+        "config": {"workload": args.workload, "nbas": m.nbas, "nao": m.nao(), "schwarz": tau,
+                   "precision": "fp64", "quartets": int(quartets), "prim_quartets": int(st["prim_quartets"]),
+                   "integrals": int(st["integrals"]), "candidates": int(st["candidates"]),
+                   "launches": int(st["launches"])},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "jk_kernel (all angular classes of one build)",
+                     "kernel_ms": jk_ms, "flops_per_launch": st["flops"]},
+    }
+    if not args.no_cpu:
+        rate, t_full, nq, used = cpu_baseline(m, P, args.cpu_seconds)
+        line["cpu_baseline"] = {
+            "value": rate, "unit": "quartets/s", "cores": 1, "kind": "port",
+            "sample": "%d random ordered shell quartets of the nbas^4 reference loop (%.1f s, oracle "
+                      "integral2e_fock restatement, 1 thread)" % (nq, used),
+            "fock_s_per_iter_extrapolated": t_full,
+            "gpu_speedup_fock": t_full / (ms_per_step * 1e-3),
+        }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

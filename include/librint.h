@@ -1,0 +1,122 @@
+/*
+ * librint-amd: C ABI of the MI355X (gfx950) two-electron path.
+ *
+ * Part 1 mirrors the reference's C boundary symbol for symbol (reference
+ * src/p2c.rs), so a caller that binds librint.so through ctypes/cgo/FFI loads
+ * this library unchanged.  Conventions kept from the reference:
+ *   - atm/bas/env are libcint arrays given as (pointer, element count); they
+ *     are copied on entry (reference src/p2c.rs:28-46);
+ *   - returned double* buffers are owned by the library and released with
+ *     free_c(ptr, exact_len) (reference src/p2c.rs:48-75);
+ *   - failures return NULL (arrays) or NaN (scalars) with the reason on
+ *     stderr (reference src/p2c.rs:166-174, 222-229).  Invalid coord/typec
+ *     codes also return NULL instead of exiting the process (the reference
+ *     calls exit(1), src/scf.rs:25-28).
+ *
+ * Part 2 adds what a GPU path needs: a persistent context (basis staged in
+ * HBM, Schwarz bounds and quartet bins computed once), the fused Fock build
+ * on host or device buffers, batched shell-quartet blocks and counters.
+ */
+#ifndef LIBRINT_AMD_H
+#define LIBRINT_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------ reference boundary ---- */
+
+/* reference src/p2c.rs:67-75 */
+void free_c(double *ptr, size_t len);
+
+/* nao*nao one-electron matrix, R[mu*nao+nu]; coord 0 cartesian (1 spherical:
+ * not supported, returns NULL); typec 0 overlap, 1 kinetic, 2 nuclear.
+ * reference src/p2c.rs:85-100, src/scf.rs:160-213 */
+double *int1e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, int coord, int typec);
+
+/* nao^4 ERI tensor R[((i*n+j)*n+k)*n+l] = (ij|kl), computed on the GPU.
+ * reference src/p2c.rs:108-123, src/scf.rs:215-311 */
+double *int2e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, int coord);
+
+/* Converged RHF density (nao*nao) or NULL; reference SCF protocol (P0 = I,
+ * Roothaan, |dP|_F/2 <= conv).  reference src/p2c.rs:152-177, src/scf.rs:743-799 */
+double *density_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, size_t nelec,
+                  int imax, double conv);
+
+/* E[P] = tr(P(T+V)) + 1/2 tr(P G[P]) + E_nuc.  reference src/p2c.rs:185-202, src/scf.rs:827-956 */
+double energy_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
+
+/* SCF energy or NaN.  reference src/p2c.rs:209-230, src/scf.rs:958-969 */
+double scf_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, size_t nelec, int imax,
+             double conv);
+
+/* Basis-parameter gradients of the reference (src/p2c.rs:130-144, 239-394).
+ * Not provided by this library yet: each returns NULL and says so on stderr,
+ * so that bindings written for the reference still load. */
+double *dS_u(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l);
+double *grad_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
+double *dS_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
+double *dHcore_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
+double *dR_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
+double *danalytical_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P,
+                      size_t P_l);
+double *denergy_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
+
+/* --------------------------------------------------------- extensions ---- */
+
+/* Stateless direct two-electron Fock matrix G = J - K/2 (nao*nao, release
+ * with free_c), the GPU counterpart of integral2e_fock (reference
+ * src/scf.rs:316-410).  schwarz_tol: quartets with Q_ij*Q_kl <= tol are
+ * skipped (0 keeps every quartet).  precision: 0 = FP64. */
+double *fock_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l,
+               int coord, int precision, double schwarz_tol);
+
+typedef struct librint_ctx librint_ctx;
+
+/* Stage a basis on the current GPU; NULL on failure. */
+librint_ctx *librint_ctx_create(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env,
+                                size_t env_l);
+void librint_ctx_destroy(librint_ctx *ctx);
+int librint_ctx_nao(const librint_ctx *ctx);
+/* Set the Schwarz threshold (default 1e-14); returns 0 on success. */
+int librint_ctx_set_schwarz(librint_ctx *ctx, double tol);
+
+/* G = J - K/2 from host buffers (nao*nao each).  Returns 0 on success. */
+int librint_fock(librint_ctx *ctx, const double *P, double *G);
+/* Same on device buffers, enqueued on `stream` (a hipStream_t, NULL = default).
+ * Asynchronous: returns after enqueueing.  Returns 0 on success. */
+int librint_fock_device(librint_ctx *ctx, const double *P_dev, double *G_dev, void *stream);
+
+/* Blocks (ij|kl) of nq shell quartets (4 ints each, indices into bas), each
+ * in cint2e_cart layout (i fastest), concatenated into out.  Returns the
+ * number of doubles written, or (size_t)-1 on failure.  With out == NULL only
+ * the size is returned. */
+size_t librint_int2e_shells(librint_ctx *ctx, const int *quartets, int nq, double *out);
+
+/* Schwarz bound sqrt(max|(ij|ij)|) per shell pair i >= j, index i*(i+1)/2+j
+ * (nbas*(nbas+1)/2 doubles). Returns 0 on success. */
+int librint_schwarz(librint_ctx *ctx, double *out);
+
+/* Counters of the current quartet plan, out[0..7]:
+ * quartets, primitive quartets, contracted integrals, algorithmic flops,
+ * candidate quartets, kernel launches, device ms of the last build, Schwarz tol.
+ * Returns the number of values written. */
+int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
+
+/* RHF on the context.  guess: 0 identity (reference), 1 core Hamiltonian,
+ * 2 caller density in P_inout.  diis: 0/1.  On return P_inout (nao*nao,
+ * may be NULL) holds the final density and *iters the cycle count.
+ * Returns the energy, NaN on failure. */
+double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess, int diis, double *P_inout,
+                   int *iters);
+
+/* Last error message of this thread ("" if none). */
+const char *librint_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,88 @@
+"""Persistent GPU context: basis staged once, Fock builds per iteration."""
+import ctypes
+
+import numpy as np
+
+from ._bindings import library
+from . import utils
+
+STAT_KEYS = ("quartets", "prim_quartets", "integrals", "flops", "candidates", "launches",
+             "kernel_ms", "schwarz_tol")
+
+
+class FockContext:
+    def __init__(self, mol, schwarz_tol=1e-14):
+        atm, bas, env, nelec = utils.prep(mol)
+        self._keep = (atm, bas, env)
+        self.nelectron = nelec
+        self.nbas = len(bas) // 8 if bas.ndim == 1 else bas.shape[0]
+        self.h = library.librint_ctx_create(*utils.mol_args(atm, bas, env))
+        if not self.h:
+            raise RuntimeError("librint context creation failed: " + utils.last_error())
+        self.nao = library.librint_ctx_nao(self.h)
+        self.set_schwarz(schwarz_tol)
+
+    def close(self):
+        if getattr(self, "h", None):
+            library.librint_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_schwarz(self, tol):
+        if library.librint_ctx_set_schwarz(self.h, float(tol)) != 0:
+            raise RuntimeError(utils.last_error())
+
+    def fock(self, P):
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        assert P.shape == (self.nao, self.nao)
+        G = np.empty_like(P)
+        dp = ctypes.POINTER(ctypes.c_double)
+        if library.librint_fock(self.h, P.ctypes.data_as(dp), G.ctypes.data_as(dp)) != 0:
+            raise RuntimeError("librint fock failed: " + utils.last_error())
+        return G
+
+    def fock_device(self, P_dev, G_dev, stream=None):
+        """P_dev, G_dev: contiguous float64 torch CUDA tensors (nao x nao)."""
+        assert P_dev.is_contiguous() and G_dev.is_contiguous()
+        st = stream.cuda_stream if stream is not None else None
+        if library.librint_fock_device(self.h, ctypes.c_void_p(P_dev.data_ptr()),
+                                       ctypes.c_void_p(G_dev.data_ptr()), ctypes.c_void_p(st)) != 0:
+            raise RuntimeError("librint fock failed: " + utils.last_error())
+
+    def shells(self, quartets):
+        """List of (i,j,k,l) shell quartets -> list of (di,dj,dk,dl) blocks."""
+        q = np.ascontiguousarray(np.asarray(quartets, dtype=np.int32).reshape(-1, 4))
+        ip = ctypes.POINTER(ctypes.c_int)
+        dp = ctypes.POINTER(ctypes.c_double)
+        n = library.librint_int2e_shells(self.h, q.ctypes.data_as(ip), len(q), None)
+        if n == ctypes.c_size_t(-1).value:
+            raise RuntimeError(utils.last_error())
+        out = np.zeros(max(n, 1))
+        library.librint_int2e_shells(self.h, q.ctypes.data_as(ip), len(q), out.ctypes.data_as(dp))
+        bas = self._keep[1].reshape(-1, 8)
+        res, off = [], 0
+        for s in q:
+            dims = [((bas[x, 1] + 1) * (bas[x, 1] + 2) // 2) * bas[x, 3] for x in s]
+            sz = int(np.prod(dims))
+            res.append(out[off:off + sz].reshape(dims[::-1]).transpose(3, 2, 1, 0).copy())
+            off += sz
+        return res
+
+    def schwarz(self):
+        out = np.zeros(self.nbas * (self.nbas + 1) // 2)
+        library.librint_schwarz(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return out
+
+    def stats(self):
+        out = np.zeros(8)
+        library.librint_ctx_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 8)
+        return dict(zip(STAT_KEYS, out.tolist()))
+
+    def scf(self, imax=200, conv=1e-6, guess=0, diis=False, P0=None):
+        P = np.zeros((self.nao, self.nao)) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
+        it = ctypes.c_int(0)
+        E = library.librint_scf(self.h, int(self.nelectron), int(imax), float(conv), int(guess), int(bool(diis)),
+                                P.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(it))
+        return E, P, it.value

@@ -1,0 +1,533 @@
+// Host side of the two-electron context: basis parsing (reference
+// src/utils.rs:17-81 layout, src/eri.rs:98-227 per-quartet constants),
+// primitive-pair staging, Schwarz bounds from the GPU diagonal (ij|ij),
+// binning and quartet task lists.
+#include "context.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <tuple>
+
+namespace librint {
+
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int cls_of(int la, int lb)
+{
+    static const int tab[3][3] = {{0, 1, 3}, {-1, 2, 4}, {-1, -1, 5}};
+    if (la > lb || lb > 2) return -1;
+    return tab[la][lb];
+}
+
+static void cls_ls(int c, int &la, int &lb)
+{
+    static const int A[6] = {0, 0, 1, 0, 1, 2}, Bv[6] = {0, 1, 1, 2, 2, 2};
+    la = A[c];
+    lb = Bv[c];
+}
+
+static double common_fac_sp(int l)
+{ // reference src/g1e.rs:776-782
+    if (l == 0) return 0.282094791773878143;
+    if (l == 1) return 0.488602511902919921;
+    return 1.0;
+}
+
+template <typename T>
+static T *dev_upload(const std::vector<T> &v)
+{
+    T *p = nullptr;
+    if (v.empty()) return nullptr;
+    check(hipMalloc(&p, sizeof(T) * v.size()), "hipMalloc");
+    check(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice), "hipMemcpy H2D");
+    return p;
+}
+
+template <typename T>
+static void dev_free(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env, size_t env_l)
+    : atm_(atm, atm + atm_l), bas_(bas, bas + bas_l), env_(env, env + env_l)
+{
+    if (atm_l % 6 || bas_l % 8) throw std::invalid_argument("atm/bas lengths must be multiples of 6/8");
+    if (env_l > 8 && env_[8] != 0.0) throw std::invalid_argument("range-separated (env[8] != 0) integrals are not supported");
+    const int nb = (int)(bas_l / 8);
+    int ao = 0;
+    for (int s = 0; s < nb; s++) {
+        const int *b = &bas_[8 * s];
+        if (b[0] < 0 || (size_t)b[0] >= atm_l / 6) throw std::invalid_argument("bas atom index out of range");
+        if (b[1] < 0 || b[1] > 2) throw std::invalid_argument("angular momentum l > 2 is not supported by the GPU path");
+        // general contraction: one segmented virtual shell per contracted function
+        for (int c = 0; c < b[3]; c++) {
+            Shell sh;
+            sh.atom = b[0];
+            sh.l = b[1];
+            sh.nprim = b[2];
+            sh.nctr = 1;
+            sh.pexp = b[5];
+            sh.pcoef = b[6] + c * b[2];
+            const int pr = atm_[6 * b[0] + 1];
+            for (int d = 0; d < 3; d++) sh.r[d] = env_[pr + d];
+            sh.nf = ncart(b[1]);
+            sh.ao = ao;
+            ao += sh.nf;
+            shells_.push_back(sh);
+        }
+    }
+    nao_ = ao;
+    check(hipEventCreate(&ev0_), "hipEventCreate");
+    check(hipEventCreate(&ev1_), "hipEventCreate");
+    build_pairs();
+    upload();
+    compute_schwarz();
+    build_tasks();
+}
+
+Context::~Context()
+{
+    dev_free(d_pair_info_);
+    dev_free(d_pair_rb_);
+    dev_free(d_pair_rab_);
+    dev_free(d_pp_);
+    dev_free(d_ao_off_);
+    dev_free(d_tasks_);
+    dev_free(d_M_);
+    dev_free(d_D_);
+    dev_free(d_G_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+}
+
+DevBasis Context::devb() const
+{
+    DevBasis B;
+    B.pair_info = d_pair_info_;
+    B.pair_rb = d_pair_rb_;
+    B.pair_rab = d_pair_rab_;
+    B.pp = d_pp_;
+    B.ao_off = d_ao_off_;
+    return B;
+}
+
+void Context::build_pairs()
+{
+    const double expcutoff = (env_[0] == 0.0) ? 60.0 : std::max(40.0, env_[0]) + 1.0;
+    const int ns = (int)shells_.size();
+    pairs_.clear();
+    pp_.clear();
+    pair_of_ij_.assign((size_t)ns * (ns + 1) / 2, -1);
+    for (int i = 0; i < ns; i++)
+        for (int j = 0; j <= i; j++) {
+            const Shell &si = shells_[i], &sj = shells_[j];
+            PairH P;
+            P.shell_i = i;
+            P.shell_j = j;
+            if (si.l <= sj.l) {
+                P.sa = i;
+                P.sb = j;
+            } else {
+                P.sa = j;
+                P.sb = i;
+            }
+            const Shell &A = shells_[P.sa], &Bs = shells_[P.sb];
+            P.cls = cls_of(A.l, Bs.l);
+            double rab2 = 0;
+            for (int d = 0; d < 3; d++) rab2 += (A.r[d] - Bs.r[d]) * (A.r[d] - Bs.r[d]);
+            // pair-level part of the primitive screen (reference src/eri.rs:192-203)
+            const double alast = env_[A.pexp + A.nprim - 1] + env_[Bs.pexp + Bs.nprim - 1];
+            double log_rr = 1.7 - 1.5 * std::log(alast);
+            if (A.l + Bs.l > 0) log_rr += (A.l + Bs.l) * std::log(std::sqrt(rab2) + 1.0);
+            const double cf = common_fac_sp(A.l) * common_fac_sp(Bs.l);
+            P.off = (int)pp_.size();
+            for (int q = 0; q < Bs.nprim; q++)
+                for (int p = 0; p < A.nprim; p++) {
+                    const double ap = env_[A.pexp + p], aq = env_[Bs.pexp + q];
+                    const double cp = env_[A.pcoef + p], cq = env_[Bs.pcoef + q];
+                    if (cp == 0.0 || cq == 0.0) continue;
+                    const double e = ap + aq;
+                    const double eij = ap * aq / e * rab2;
+                    const double cce = eij - log_rr - std::log(std::fabs(cp)) - std::log(std::fabs(cq));
+                    if (cce > expcutoff) continue;
+                    PrimPair r;
+                    r.e = e;
+                    r.K = cp * cq * cf * std::exp(-eij);
+                    r.pbx = (ap * A.r[0] + aq * Bs.r[0]) / e - Bs.r[0];
+                    r.pby = (ap * A.r[1] + aq * Bs.r[1]) / e - Bs.r[1];
+                    r.pbz = (ap * A.r[2] + aq * Bs.r[2]) / e - Bs.r[2];
+                    r.pad0 = r.pad1 = r.pad2 = 0.0;
+                    pp_.push_back(r);
+                }
+            P.npp = (int)pp_.size() - P.off;
+            P.Q = 0.0;
+            if (P.npp == 0) continue;
+            pair_of_ij_[(size_t)i * (i + 1) / 2 + j] = (int)pairs_.size();
+            pairs_.push_back(P);
+        }
+}
+
+void Context::upload()
+{
+    dev_free(d_pair_info_);
+    dev_free(d_pair_rb_);
+    dev_free(d_pair_rab_);
+    dev_free(d_ao_off_);
+    std::vector<int4> info(pairs_.size());
+    std::vector<double4> rb(pairs_.size()), rab(pairs_.size());
+    for (size_t p = 0; p < pairs_.size(); p++) {
+        const PairH &P = pairs_[p];
+        const Shell &A = shells_[P.sa], &Bs = shells_[P.sb];
+        info[p] = make_int4(P.sa, P.sb, P.off, P.npp);
+        rb[p] = make_double4(Bs.r[0], Bs.r[1], Bs.r[2], P.Q);
+        rab[p] = make_double4(Bs.r[0] - A.r[0], Bs.r[1] - A.r[1], Bs.r[2] - A.r[2], 0.0);
+    }
+    std::vector<int> aoo(shells_.size());
+    for (size_t s = 0; s < shells_.size(); s++) aoo[s] = shells_[s].ao;
+    d_pair_info_ = dev_upload(info);
+    d_pair_rb_ = dev_upload(rb);
+    d_pair_rab_ = dev_upload(rab);
+    d_ao_off_ = dev_upload(aoo);
+    if (!d_pp_) d_pp_ = dev_upload(pp_);
+}
+
+void Context::compute_schwarz()
+{
+    // diagonal (ij|ij) per pair on the GPU, grouped by class
+    const int np = (int)pairs_.size();
+    std::vector<ListTask> tasks;
+    std::vector<std::pair<int, int>> ranges(6, {0, 0});
+    for (int c = 0; c < 6; c++) {
+        ranges[c].first = (int)tasks.size();
+        for (int p = 0; p < np; p++)
+            if (pairs_[p].cls == c) tasks.push_back(ListTask{p, p, (long long)p});
+        ranges[c].second = (int)tasks.size() - ranges[c].first;
+    }
+    ListTask *dt = dev_upload(tasks);
+    double *dq = nullptr;
+    check(hipMalloc(&dq, sizeof(double) * std::max(np, 1)), "hipMalloc");
+    const DevBasis B = devb();
+    for (int c = 0; c < 6; c++) {
+        if (!ranges[c].second) continue;
+        int la, lb;
+        cls_ls(c, la, lb);
+        check(launch_list(la, lb, la, lb, dt + ranges[c].first, ranges[c].second, B, dq, 1, nao_, 0), "schwarz launch");
+    }
+    std::vector<double> Q(np);
+    check(hipMemcpy(Q.data(), dq, sizeof(double) * np, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    dev_free(dq);
+    dev_free(dt);
+    for (int p = 0; p < np; p++) pairs_[p].Q = Q[p];
+
+    // reorder into bins: (class, npp) ascending, Q descending inside a bin
+    std::vector<int> order(np);
+    for (int p = 0; p < np; p++) order[p] = p;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        const PairH &a = pairs_[x], &b = pairs_[y];
+        if (a.cls != b.cls) return a.cls < b.cls;
+        if (a.npp != b.npp) return a.npp < b.npp;
+        return a.Q > b.Q;
+    });
+    std::vector<PairH> sorted(np);
+    for (int p = 0; p < np; p++) sorted[p] = pairs_[order[p]];
+    pairs_.swap(sorted);
+    std::fill(pair_of_ij_.begin(), pair_of_ij_.end(), -1);
+    bins_.clear();
+    for (int p = 0; p < np; p++) {
+        const PairH &P = pairs_[p];
+        pair_of_ij_[(size_t)P.shell_i * (P.shell_i + 1) / 2 + P.shell_j] = p;
+        if (bins_.empty() || bins_.back().cls != P.cls || bins_.back().npp != P.npp)
+            bins_.push_back(Bin{P.cls, P.npp, p, 0});
+        bins_.back().count++;
+    }
+    upload();
+}
+
+void Context::set_schwarz(double tau)
+{
+    tau_ = tau;
+    build_tasks();
+}
+
+// Per primitive quartet flop count of the fp64 kernel's algorithm (the
+// definition used for roofline numbers): root evaluation, recurrence
+// coefficients, VRR, HRR and the gout products, FMA counted as 2 flops.
+static double flops_prim(int la, int lb, int lc, int ld)
+{
+    const int nmax = la + lb, mmax = lc + ld, nr = (nmax + mmax) / 2 + 1;
+    const int dk = la + 1, dl = dk * (lc + 1);
+    (void)dl;
+    const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+    double setup = 30;                 // exponents, P-Q, x, prefactor (incl. sqrt/div)
+    double roots = nr * 2.0 * (11 * 2 + 2) + 4;  // 2n Clenshaw of degree 11
+    double coef = 16;                  // b00 b10 b01 ta tc, c00 c0p (x3), w*fac
+    double vrr = 0;
+    if (nmax > 0) vrr += 1 + 4.0 * (nmax - 1);
+    if (mmax > 0) {
+        vrr += 1 + 4.0 * (mmax - 1);
+        if (nmax > 0) vrr += 3 + 6.0 * (mmax - 1);
+    }
+    if (nmax > 1) vrr += (double)mmax * (nmax - 1) * 7.0;
+    double hrr = 0;
+    for (int i = 1; i <= la; i++) hrr += 2.0 * (nmax - i + 1) * (mmax + 1);
+    for (int k = 1; k <= lc; k++) hrr += 2.0 * (lb + 1) * (mmax - k + 1) * dk;
+    const double gout = 3.0 * nf;  // 2 mul + 1 add
+    return setup + roots + nr * (coef + 3 * (vrr + hrr) + gout);
+}
+
+void Context::build_tasks()
+{
+    const int nb = (int)bins_.size();
+    std::vector<JKTask> tasks;
+    launches_.clear();
+    stats_ = Stats();
+    const long long np = (long long)pairs_.size();
+    stats_.candidates = np * (np + 1) / 2;
+    // group bin pairs by class pair
+    std::map<std::pair<int, int>, std::vector<std::tuple<long long, JKTask>>> by_cls;
+    for (int X = 0; X < nb; X++)
+        for (int Y = 0; Y <= X; Y++) {
+            const Bin &bx = bins_[X], &by = bins_[Y];
+            int la, lb, lc, ld;
+            cls_ls(bx.cls, la, lb);
+            cls_ls(by.cls, lc, ld);
+            if (!class_supported(la, lb, lc, ld))
+                throw std::runtime_error("angular class not supported by the GPU kernels");
+            const double fp = flops_prim(la, lb, lc, ld);
+            const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+            auto &lst = by_cls[{bx.cls, by.cls}];
+            for (int i = 0; i < bx.count; i++) {
+                const int p = bx.first + i;
+                const double Qp = pairs_[p].Q;
+                // kets of bin Y with Q_p Q_k > tau: a prefix (Q descending)
+                int lo = 0, hi = by.count;
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (Qp * pairs_[by.first + mid].Q > tau_) lo = mid + 1;
+                    else hi = mid;
+                }
+                int n = lo;
+                if (X == Y) n = std::min(n, i + 1);
+                if (n <= 0) continue;
+                stats_.quartets += n;
+                stats_.prim_quartets += (long long)n * bx.npp * by.npp;
+                stats_.integrals += (long long)n * nf;
+                stats_.flops += (double)n * bx.npp * by.npp * fp + 12.0 * nf * n;
+                for (int k0 = 0; k0 < n; k0 += 64) {
+                    JKTask t{p, by.first + k0, std::min(64, n - k0), 0};
+                    lst.emplace_back((long long)bx.npp * by.npp, t);
+                }
+            }
+        }
+    for (auto &kv : by_cls) {
+        auto &lst = kv.second;
+        // heavy tasks first for load balance
+        std::stable_sort(lst.begin(), lst.end(),
+                         [](const std::tuple<long long, JKTask> &a, const std::tuple<long long, JKTask> &b) {
+                             return std::get<0>(a) > std::get<0>(b);
+                         });
+        ClassLaunch L;
+        cls_ls(kv.first.first, L.la, L.lb);
+        cls_ls(kv.first.second, L.lc, L.ld);
+        L.first = (int)tasks.size();
+        for (auto &e : lst) tasks.push_back(std::get<1>(e));
+        L.count = (int)tasks.size() - L.first;
+        if (L.count) launches_.push_back(L);
+    }
+    stats_.launches = (int)launches_.size();
+    dev_free(d_tasks_);
+    d_tasks_ = dev_upload(tasks);
+}
+
+void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
+{
+    const size_t nn = (size_t)nao_ * nao_;
+    if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
+    check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
+    const DevBasis B = devb();
+    check(hipEventRecord(ev0_, st), "hipEventRecord");
+    for (const ClassLaunch &L : launches_)
+        check(launch_jk(L.la, L.lb, L.lc, L.ld, d_tasks_ + L.first, L.count, B, D_dev, d_M_, nao_, st), "jk launch");
+    check(hipEventRecord(ev1_, st), "hipEventRecord");
+    check(launch_symmetrize(d_M_, G_dev, nao_, st), "symmetrize launch");
+}
+
+void Context::fock_host(const double *D, double *G)
+{
+    const size_t nn = (size_t)nao_ * nao_;
+    if (!d_D_) check(hipMalloc(&d_D_, sizeof(double) * nn), "hipMalloc");
+    if (!d_G_) check(hipMalloc(&d_G_, sizeof(double) * nn), "hipMalloc");
+    check(hipMemcpy(d_D_, D, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    fock_device(d_D_, d_G_, 0);
+    check(hipMemcpy(G, d_G_, sizeof(double) * nn, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    check(hipEventElapsedTime(&stats_.last_kernel_ms, ev0_, ev1_), "hipEventElapsedTime");
+}
+
+void Context::tensor_host(double *R)
+{
+    const size_t n = (size_t)nao_;
+    const size_t n4 = n * n * n * n;
+    double *dR = nullptr;
+    check(hipMalloc(&dR, sizeof(double) * n4), "hipMalloc");
+    check(hipMemset(dR, 0, sizeof(double) * n4), "hipMemset");
+    // every canonical pair of pairs, no screening (exact tensor)
+    const int np = (int)pairs_.size();
+    std::map<std::pair<int, int>, std::vector<ListTask>> by_cls;
+    for (int p = 0; p < np; p++)
+        for (int q = 0; q <= p; q++) {
+            int a = p, b = q;
+            if (pairs_[a].cls < pairs_[b].cls) std::swap(a, b);
+            by_cls[{pairs_[a].cls, pairs_[b].cls}].push_back(ListTask{a, b, 0});
+        }
+    const DevBasis B = devb();
+    std::vector<ListTask *> bufs;
+    for (auto &kv : by_cls) {
+        int la, lb, lc, ld;
+        cls_ls(kv.first.first, la, lb);
+        cls_ls(kv.first.second, lc, ld);
+        ListTask *dt = dev_upload(kv.second);
+        bufs.push_back(dt);
+        check(launch_list(la, lb, lc, ld, dt, (int)kv.second.size(), B, dR, 2, nao_, 0), "tensor launch");
+    }
+    check(hipMemcpy(R, dR, sizeof(double) * n4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    for (auto p : bufs) dev_free(p);
+    dev_free(dR);
+}
+
+size_t Context::shells_host(const int *quartets, int nq, double *out)
+{
+    // Each requested (I,J,K,L) maps to a canonical stored pair of pairs; the
+    // block is computed in stored orientation and permuted back here.
+    struct Req {
+        int bra, ket;
+        bool swap_bk;
+        long long off_dev;
+        long long off_out;
+        int dims[4];
+    };
+    const int nb = nbas();
+    std::vector<int> cum(bas_.size() / 8 + 1, 0);  // first virtual shell of each input shell
+    {
+        int v = 0;
+        for (size_t s = 0; s < bas_.size() / 8; s++) {
+            cum[s] = v;
+            v += bas_[8 * s + 3];
+        }
+        cum[bas_.size() / 8] = v;
+    }
+    (void)nb;
+    std::vector<Req> reqs;
+    std::map<std::pair<int, int>, std::vector<ListTask>> by_cls;
+    long long dev_total = 0, out_total = 0;
+    for (int t = 0; t < nq; t++) {
+        const int *s = quartets + 4 * t;
+        for (int a = 0; a < 4; a++)
+            if (s[a] < 0 || (size_t)s[a] >= bas_.size() / 8) throw std::invalid_argument("shell index out of range");
+        if (bas_[8 * s[0] + 3] != 1 || bas_[8 * s[1] + 3] != 1 || bas_[8 * s[2] + 3] != 1 || bas_[8 * s[3] + 3] != 1)
+            throw std::invalid_argument("shell blocks support segmented shells only");
+        const int I = cum[s[0]], J = cum[s[1]], K = cum[s[2]], L = cum[s[3]];
+        auto pidx = [&](int x, int y) {
+            if (x < y) std::swap(x, y);
+            return pair_of_ij_[(size_t)x * (x + 1) / 2 + y];
+        };
+        Req r;
+        r.bra = pidx(I, J);
+        r.ket = pidx(K, L);
+        for (int a = 0; a < 4; a++) r.dims[a] = shells_[cum[s[a]]].nf;
+        r.off_out = out_total;
+        out_total += (long long)r.dims[0] * r.dims[1] * r.dims[2] * r.dims[3];
+        r.swap_bk = false;
+        r.off_dev = -1;
+        if (r.bra >= 0 && r.ket >= 0) {
+            if (pairs_[r.bra].cls < pairs_[r.ket].cls) {
+                std::swap(r.bra, r.ket);
+                r.swap_bk = true;
+            }
+            r.off_dev = dev_total;
+            dev_total += (long long)r.dims[0] * r.dims[1] * r.dims[2] * r.dims[3];
+            by_cls[{pairs_[r.bra].cls, pairs_[r.ket].cls}].push_back(ListTask{r.bra, r.ket, r.off_dev});
+        }
+        reqs.push_back(r);
+    }
+    std::vector<double> blocks(std::max<long long>(dev_total, 1));
+    if (dev_total) {
+        double *dout = nullptr;
+        check(hipMalloc(&dout, sizeof(double) * dev_total), "hipMalloc");
+        const DevBasis B = devb();
+        std::vector<ListTask *> bufs;
+        for (auto &kv : by_cls) {
+            int la, lb, lc, ld;
+            cls_ls(kv.first.first, la, lb);
+            cls_ls(kv.first.second, lc, ld);
+            ListTask *dt = dev_upload(kv.second);
+            bufs.push_back(dt);
+            check(launch_list(la, lb, lc, ld, dt, (int)kv.second.size(), B, dout, 0, nao_, 0), "shells launch");
+        }
+        check(hipMemcpy(blocks.data(), dout, sizeof(double) * dev_total, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        for (auto p : bufs) dev_free(p);
+        dev_free(dout);
+    }
+    for (int t = 0; t < nq; t++) {
+        const Req &r = reqs[t];
+        const int *s = quartets + 4 * t;
+        const int d0 = r.dims[0], d1 = r.dims[1], d2 = r.dims[2], d3 = r.dims[3];
+        double *o = out + r.off_out;
+        if (r.off_dev < 0) {
+            std::memset(o, 0, sizeof(double) * d0 * d1 * d2 * d3);
+            continue;
+        }
+        // stored block: pair X = (sa, sb), pair Y = (sc, sd), a fastest
+        const PairH &X = pairs_[r.bra], &Y = pairs_[r.ket];
+        const int na = shells_[X.sa].nf, nbf = shells_[X.sb].nf, nc = shells_[Y.sa].nf;
+        const int I = cum[s[0]], J = cum[s[1]], K = cum[s[2]], L = cum[s[3]];
+        for (int l = 0; l < d3; l++)
+            for (int k = 0; k < d2; k++)
+                for (int j = 0; j < d1; j++)
+                    for (int i = 0; i < d0; i++) {
+                        // component of each stored slot
+                        int fi[4];
+                        const int sh[4] = {I, J, K, L};
+                        const int fv[4] = {i, j, k, l};
+                        // bra of stored block = requested bra unless swapped
+                        const int b0 = r.swap_bk ? 2 : 0, k0 = r.swap_bk ? 0 : 2;
+                        // stored bra pair (sa, sb) from requested slots b0, b0+1
+                        if (sh[b0] == X.sa && (sh[b0 + 1] == X.sb)) {
+                            fi[0] = fv[b0];
+                            fi[1] = fv[b0 + 1];
+                        } else {
+                            fi[0] = fv[b0 + 1];
+                            fi[1] = fv[b0];
+                        }
+                        if (sh[k0] == Y.sa && (sh[k0 + 1] == Y.sb)) {
+                            fi[2] = fv[k0];
+                            fi[3] = fv[k0 + 1];
+                        } else {
+                            fi[2] = fv[k0 + 1];
+                            fi[3] = fv[k0];
+                        }
+                        o[i + d0 * (j + d1 * (k + d2 * l))] =
+                            blocks[r.off_dev + fi[0] + na * (fi[1] + nbf * (fi[2] + nc * fi[3]))];
+                    }
+    }
+    return (size_t)out_total;
+}
+
+std::vector<double> Context::schwarz_pairs() const
+{
+    const int ns = (int)shells_.size();
+    std::vector<double> q((size_t)ns * (ns + 1) / 2, 0.0);
+    for (size_t k = 0; k < q.size(); k++)
+        if (pair_of_ij_[k] >= 0) q[k] = pairs_[pair_of_ij_[k]].Q;
+    return q;
+}
+
+}  // namespace librint

@@ -1,0 +1,116 @@
+// Persistent two-electron context: basis staged in HBM once, Schwarz bounds
+// and the binned quartet task lists computed once, then one fused J/K build
+// per SCF iteration.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "eri_core.h"
+#include "kernels.h"
+
+namespace librint {
+
+struct Shell {
+    int atom, l, nprim, nctr, pexp, pcoef;
+    double r[3];
+    int ao;  // first cartesian AO
+    int nf;  // cartesian functions per contraction
+};
+
+struct Stats {
+    long long quartets = 0;        // canonical shell quartets evaluated per build
+    long long prim_quartets = 0;   // primitive quartets evaluated per build
+    long long integrals = 0;       // contracted cartesian integrals per build
+    double flops = 0;              // algorithmic flops per build (fp64 path)
+    long long candidates = 0;      // canonical shell quartets before screening
+    int launches = 0;
+    float last_kernel_ms = 0.f;    // device time of the last build (events)
+};
+
+struct ClassLaunch {
+    int la, lb, lc, ld;
+    int first, count;  // range in the device task array
+};
+
+class Context {
+public:
+    Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env, size_t env_l);
+    ~Context();
+
+    int nao() const { return nao_; }
+    int nbas() const { return (int)shells_.size(); }
+    int natm() const { return (int)(atm_.size() / 6); }
+
+    // Screening threshold on Q_ij * Q_kl; rebuilds the task lists.
+    void set_schwarz(double tau);
+    double schwarz() const { return tau_; }
+
+    // G = J - K/2 for a symmetric density (device pointers, nao*nao, row-major).
+    void fock_device(const double *D_dev, double *G_dev, hipStream_t st);
+    // Host-pointer convenience wrapper.
+    void fock_host(const double *D, double *G);
+
+    // Full nao^4 tensor (host buffer, R[((i*n+j)*n+k)*n+l]).
+    void tensor_host(double *R);
+    // Blocks for a list of shell quartets, each in cint2e_cart layout
+    // (i fastest), concatenated; returns the number of doubles written.
+    size_t shells_host(const int *quartets, int nq, double *out);
+    // Schwarz bound per shell pair (i >= j), host copy, index i*(i+1)/2 + j.
+    std::vector<double> schwarz_pairs() const;
+
+    const Stats &stats() const { return stats_; }
+    const std::vector<Shell> &shells() const { return shells_; }
+    const std::vector<int> &atm() const { return atm_; }
+    const std::vector<double> &env() const { return env_; }
+    const std::vector<int> &bas() const { return bas_; }
+    const std::vector<ClassLaunch> &launches() const { return launches_; }
+
+private:
+    void build_pairs();
+    void compute_schwarz();
+    void build_tasks();
+    void upload();
+
+    std::vector<int> atm_, bas_;
+    std::vector<double> env_;
+    std::vector<Shell> shells_;
+    int nao_ = 0;
+    double tau_ = 1e-14;
+
+    // pair list in bin order (bins = (class, npp), Q descending inside a bin)
+    struct PairH {
+        int sa, sb;   // sa: HRR target (la <= lb), sb: VRR centre
+        int shell_i, shell_j;  // canonical i >= j
+        int cls, npp, off;
+        double Q;
+    };
+    std::vector<PairH> pairs_;
+    std::vector<PrimPair> pp_;
+    std::vector<int> pair_of_ij_;  // canonical (i,j) -> position in pairs_
+    struct Bin {
+        int cls, npp, first, count;
+    };
+    std::vector<Bin> bins_;
+
+    std::vector<ClassLaunch> launches_;
+    Stats stats_;
+
+    // device
+    int4 *d_pair_info_ = nullptr;
+    double4 *d_pair_rb_ = nullptr, *d_pair_rab_ = nullptr;
+    PrimPair *d_pp_ = nullptr;
+    int *d_ao_off_ = nullptr;
+    JKTask *d_tasks_ = nullptr;
+    double *d_M_ = nullptr, *d_D_ = nullptr, *d_G_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    DevBasis devb() const;
+};
+
+int cls_of(int la, int lb);  // pair class index, la <= lb
+void check(hipError_t e, const char *what);
+
+}  // namespace librint

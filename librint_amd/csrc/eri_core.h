@@ -1,0 +1,193 @@
+// Rys-quadrature ERI core for one contracted shell quartet, fully unrolled for
+// a compile-time angular class (LA LB | LC LD).
+//
+// Convention (same recurrence as reference src/eri.rs:263-399, the lj form of
+// libcint's CINTg0_2e_2d + CINTg0_lj2d_4d, src/g2e.rs:316-606):
+//   * the 2D VRR builds momentum on centre b (bra) and d (ket) up to
+//     NMAX = LA+LB and MMAX = LC+LD;
+//   * the HRR moves it b -> a and d -> c.
+// Pairs are stored with LA <= LB and LC <= LD, so the HRR always moves the
+// smaller momentum.
+//
+// Per-root g buffers, index  i + DK*k + DL*l + DJ*j  (i on a, k on c, l on d,
+// j on b), DK = LA+1, DL = DK*(LC+1), DJ = DL*(MMAX+1).
+//
+// The contracted block is accumulated into eri[NF] with
+//   f = fa + NFA*(fb + NFB*(fc + NFC*fd))   (a fastest, cint2e_cart order).
+#pragma once
+
+#include "rys_eval.h"
+
+namespace librint {
+
+__host__ __device__ constexpr int ncart(int l) { return (l + 1) * (l + 2) / 2; }
+
+// cartesian powers of component idx of shell l, x-major descending
+// (reference src/cint_bas.rs:234-255)
+__host__ __device__ constexpr int cart_x(int l, int idx)
+{
+    int n = 0;
+    for (int lx = l; lx >= 0; lx--)
+        for (int ly = l - lx; ly >= 0; ly--) {
+            if (n == idx) return lx;
+            n++;
+        }
+    return 0;
+}
+__host__ __device__ constexpr int cart_y(int l, int idx)
+{
+    int n = 0;
+    for (int lx = l; lx >= 0; lx--)
+        for (int ly = l - lx; ly >= 0; ly--) {
+            if (n == idx) return ly;
+            n++;
+        }
+    return 0;
+}
+__host__ __device__ constexpr int cart_z(int l, int idx) { return l - cart_x(l, idx) - cart_y(l, idx); }
+
+template <int LA, int LB, int LC, int LD>
+struct Class {
+    static constexpr int NMAX = LA + LB;
+    static constexpr int MMAX = LC + LD;
+    static constexpr int NROOTS = (NMAX + MMAX) / 2 + 1;
+    static constexpr int DK = LA + 1;
+    static constexpr int DL = DK * (LC + 1);
+    static constexpr int DJ = DL * (MMAX + 1);
+    static constexpr int GS = DJ * (NMAX + 1);
+    static constexpr int NFA = ncart(LA), NFB = ncart(LB), NFC = ncart(LC), NFD = ncart(LD);
+    static constexpr int NF = NFA * NFB * NFC * NFD;
+};
+
+// 2D VRR on one axis (reference src/eri.rs:318-370)
+template <int NMAX, int MMAX, int DL, int DJ, typename T>
+__device__ __forceinline__ void vrr_axis(T *g, T cn, T cm, T b10, T b01, T b00)
+{
+    if constexpr (NMAX > 0) {
+        g[DJ] = cn * g[0];
+#pragma unroll
+        for (int n = 1; n < NMAX; n++) g[(n + 1) * DJ] = cn * g[n * DJ] + (T)n * b10 * g[(n - 1) * DJ];
+    }
+    if constexpr (MMAX > 0) {
+        g[DL] = cm * g[0];
+#pragma unroll
+        for (int m = 1; m < MMAX; m++) g[(m + 1) * DL] = cm * g[m * DL] + (T)m * b01 * g[(m - 1) * DL];
+        if constexpr (NMAX > 0) {
+            g[DJ + DL] = cm * g[DJ] + b00 * g[0];
+#pragma unroll
+            for (int m = 1; m < MMAX; m++)
+                g[DJ + (m + 1) * DL] = cm * g[DJ + m * DL] + (T)m * b01 * g[DJ + (m - 1) * DL] + b00 * g[m * DL];
+        }
+    }
+    if constexpr (NMAX > 1) {
+#pragma unroll
+        for (int m = 1; m <= MMAX; m++) {
+#pragma unroll
+            for (int n = 1; n < NMAX; n++)
+                g[m * DL + (n + 1) * DJ] =
+                    cn * g[m * DL + n * DJ] + (T)n * b10 * g[m * DL + (n - 1) * DJ] + (T)m * b00 * g[(m - 1) * DL + n * DJ];
+        }
+    }
+}
+
+// HRR on one axis: b -> a with rab = Rb - Ra, d -> c with rcd = Rd - Rc
+// (reference src/eri.rs:372-398)
+template <int LA, int LB, int LC, int NMAX, int MMAX, int DK, int DL, int DJ, typename T>
+__device__ __forceinline__ void hrr_axis(T *g, T rab, T rcd)
+{
+#pragma unroll
+    for (int i = 1; i <= LA; i++)
+#pragma unroll
+        for (int j = 0; j <= NMAX - i; j++)
+#pragma unroll
+            for (int l = 0; l <= MMAX; l++) {
+                const int p = j * DJ + l * DL + i;
+                g[p] = rab * g[p - 1] + g[p - 1 + DJ];
+            }
+#pragma unroll
+    for (int j = 0; j <= LB; j++)
+#pragma unroll
+        for (int k = 1; k <= LC; k++)
+#pragma unroll
+            for (int l = 0; l <= MMAX - k; l++) {
+                const int p = j * DJ + l * DL + k * DK;
+#pragma unroll
+                for (int n = 0; n < DK; n++) g[p + n] = rcd * g[p + n - DK] + g[p + n - DK + DL];
+            }
+}
+
+// Accumulate sum_r gx*gy*gz of one root into the contracted block.
+template <int LA, int LB, int LC, int LD, typename T, typename A>
+__device__ __forceinline__ void gout_acc(A *eri, const T *gx, const T *gy, const T *gz)
+{
+    using C = Class<LA, LB, LC, LD>;
+#pragma unroll
+    for (int fd = 0; fd < C::NFD; fd++)
+#pragma unroll
+        for (int fc = 0; fc < C::NFC; fc++)
+#pragma unroll
+            for (int fb = 0; fb < C::NFB; fb++)
+#pragma unroll
+                for (int fa = 0; fa < C::NFA; fa++) {
+                    const int ix = cart_x(LB, fb) * C::DJ + cart_x(LD, fd) * C::DL + cart_x(LC, fc) * C::DK + cart_x(LA, fa);
+                    const int iy = cart_y(LB, fb) * C::DJ + cart_y(LD, fd) * C::DL + cart_y(LC, fc) * C::DK + cart_y(LA, fa);
+                    const int iz = cart_z(LB, fb) * C::DJ + cart_z(LD, fd) * C::DL + cart_z(LC, fc) * C::DK + cart_z(LA, fa);
+                    const int f = fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd));
+                    eri[f] += (A)(gx[ix] * gy[iy] * gz[iz]);
+                }
+}
+
+// Primitive-pair record (one per surviving primitive pair of a shell pair).
+//   e = ai + aj, K = ca*cb*fac_sp(la)*fac_sp(lb)*exp(-ai aj/e |Ra-Rb|^2),
+//   pb = P - Rb  (P the Gaussian product centre, Rb the VRR centre).
+struct PrimPair {
+    double e;
+    double K;
+    double pbx, pby, pbz;
+    double pad0, pad1, pad2;
+};
+
+// One primitive quartet, all roots, accumulated into eri (double precision).
+// rbd = Rb - Rd, rab = Rb - Ra, rcd = Rd - Rc.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p, const PrimPair &q, const double rbd[3],
+                                                  const double rab[3], const double rcd[3])
+{
+    using C = Class<LA, LB, LC, LD>;
+    constexpr double TWO_PI_52 = 34.98683665524972497; // 2 pi^{5/2}
+    const double aij = p.e, akl = q.e;
+    const double a1 = aij * akl;
+    const double rs = 1.0 / (aij + akl);
+    const double a0 = a1 * rs;
+    const double pq[3] = {p.pbx - q.pbx + rbd[0], p.pby - q.pby + rbd[1], p.pbz - q.pbz + rbd[2]};
+    const double rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
+    const double fac1 = TWO_PI_52 * p.K * q.K * sqrt(rs) / a1;
+    double v[C::NROOTS], w[C::NROOTS];
+    rys_nodes<C::NROOTS>(a0 * rr, v, w);
+#pragma unroll
+    for (int r = 0; r < C::NROOTS; r++) {
+        // v = t^2;  b00 = t^2/(2(aij+akl)), b10 = (1 - akl t^2/(aij+akl))/(2 aij), ...
+        const double t2 = v[r];
+        const double tr = t2 * rs;
+        const double b00 = 0.5 * tr;
+        const double b10 = 0.5 / aij * (1.0 - akl * tr);
+        const double b01 = 0.5 / akl * (1.0 - aij * tr);
+        const double ta = akl * tr; // weight of (P-Q) in the bra shift
+        const double tc = aij * tr;
+        double gx[C::GS], gy[C::GS], gz[C::GS];
+        gx[0] = 1.0;
+        gy[0] = 1.0;
+        gz[0] = w[r] * fac1;
+        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gx, p.pbx - ta * pq[0], q.pbx + tc * pq[0], b10, b01, b00);
+        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gy, p.pby - ta * pq[1], q.pby + tc * pq[1], b10, b01, b00);
+        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gz, p.pbz - ta * pq[2], q.pbz + tc * pq[2], b10, b01, b00);
+        if constexpr (LA > 0 || LC > 0) {
+            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gx, rab[0], rcd[0]);
+            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gy, rab[1], rcd[1]);
+            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gz, rab[2], rcd[2]);
+        }
+        gout_acc<LA, LB, LC, LD>(eri, gx, gy, gz);
+    }
+}
+
+} // namespace librint

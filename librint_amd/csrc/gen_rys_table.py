@@ -1,0 +1,197 @@
+"""Generate the piecewise-Chebyshev Rys quadrature table used by the HIP kernels.
+
+Rys quadrature of order n integrates f(t^2) exp(-x t^2) dt over t in [0, 1]
+exactly for polynomials f of degree < 2n:
+
+    int_0^1 f(t^2) e^{-x t^2} dt = sum_r w_r f(v_r),    v_r = t_r^2.
+
+The moments of that measure in v are the Boys functions F_k(x), k < 2n.  For
+every x the nodes v_r and weights w_r are obtained in 50-digit arithmetic:
+moments -> three-term recurrence (Chebyshev algorithm) -> Jacobi matrix ->
+symmetric eigenproblem (Golub-Welsch).  The same convention is used by
+libcint's CINTrys_roots (reference src/rys_roots.rs:3036-3064), which returns
+u = v/(1-v) and the plain weights w with sum_r w_r = F_0(x).
+
+Range [0, XCUT(n)) is split into unit-length segments; on each one every v_r and
+w_r is a degree-DEG Chebyshev series in the local variable s in [-1, 1].  Above
+XCUT(n) the asymptotic Gauss-Hermite limit is exact to double precision:
+v_r = h_r/x, w_r = H_r/sqrt(x), with h_r the squared positive nodes and H_r the
+matching weights of the 2n-point Gauss-Hermite rule (weight e^{-s^2}).
+
+Table layout (one block per n, GPU friendly: one lane runs all 2n Clenshaw
+recurrences of its segment with contiguous loads):
+    RYS_CHEB[off(n) + (seg*(DEG+1) + c)*2n + j],  j < n: v_j, j >= n: w_{j-n}
+
+Run:  python librint_amd/csrc/gen_rys_table.py   (writes rys_table.inc next to it)
+The output is committed; the generator needs mpmath and is not run at build time.
+"""
+import os
+import sys
+import multiprocessing as mpr
+
+import mpmath as mp
+
+NMAX = 9          # roots up to an all-g quartet (l <= 4)
+DEG = 11          # Chebyshev degree per segment
+SEGLEN = 1.0      # segment length in x
+DPS = 50
+
+
+def boys(m, x):
+    if x == 0:
+        return mp.mpf(1) / (2 * m + 1)
+    return mp.hyp1f1(m + mp.mpf(1) / 2, m + mp.mpf(3) / 2, -x) / (2 * m + 1)
+
+
+def rys_exact(n, x):
+    """Nodes v (ascending) and weights w of the n-point Rys rule at x."""
+    mom = [boys(k, x) for k in range(2 * n)]
+    alpha = [mp.mpf(0)] * n
+    beta = [mp.mpf(0)] * n
+    sig_prev = [mp.mpf(0)] * (2 * n)
+    sig = list(mom)
+    alpha[0] = mom[1] / mom[0]
+    beta[0] = mom[0]
+    for k in range(1, n):
+        sig_new = [mp.mpf(0)] * (2 * n)
+        for l in range(k, 2 * n - k):
+            sig_new[l] = sig[l + 1] - alpha[k - 1] * sig[l] - beta[k - 1] * sig_prev[l]
+        alpha[k] = sig_new[k + 1] / sig_new[k] - sig[k] / sig[k - 1]
+        beta[k] = sig_new[k] / sig[k - 1]
+        sig_prev, sig = sig, sig_new
+    J = mp.zeros(n, n)
+    for i in range(n):
+        J[i, i] = alpha[i]
+        if i + 1 < n:
+            J[i, i + 1] = J[i + 1, i] = mp.sqrt(beta[i + 1])
+    E, Q = mp.eigsy(J)
+    idx = sorted(range(n), key=lambda i: E[i])
+    return [E[i] for i in idx], [beta[0] * Q[0, i] ** 2 for i in idx]
+
+
+def hermite_half(n):
+    """Squared positive nodes and weights of the 2n-point Gauss-Hermite rule."""
+    nodes, weights = [], []
+    # roots of H_{2n} by polishing mpmath's polynomial roots
+    coeffs = mp.taylor(lambda s: mp.hermite(2 * n, s), 0, 2 * n)[::-1]
+    roots = sorted(r.real for r in mp.polyroots(coeffs, maxsteps=500, extraprec=200) if r.real > 0)
+    for r in roots:
+        r = mp.findroot(lambda s: mp.hermite(2 * n, s), r)
+        # w = 2^{N-1} N! sqrt(pi) / (N^2 H_{N-1}(r)^2), N = 2n
+        N = 2 * n
+        w = mp.mpf(2) ** (N - 1) * mp.factorial(N) * mp.sqrt(mp.pi) / (N * N * mp.hermite(N - 1, r) ** 2)
+        nodes.append(r * r)
+        weights.append(w)
+    return nodes, weights
+
+
+def asymptotic(n, x, hh):
+    h, H = hh
+    return [hr / x for hr in h], [Hr / mp.sqrt(x) for Hr in H]
+
+
+def cheb_fit(job):
+    n, seg = job
+    mp.mp.dps = DPS
+    c = mp.mpf(seg) * SEGLEN + mp.mpf(SEGLEN) / 2
+    half = mp.mpf(SEGLEN) / 2
+    nodes = [mp.cos(mp.pi * (j + mp.mpf(1) / 2) / (DEG + 1)) for j in range(DEG + 1)]
+    vals = [rys_exact(n, c + half * t) for t in nodes]
+    out = []
+    for c_idx in range(DEG + 1):
+        row = []
+        for which in (0, 1):
+            for r in range(n):
+                f = [vals[j][which][r] for j in range(DEG + 1)]
+                s = 2 * mp.fsum(f[j] * mp.cos(mp.pi * c_idx * (j + mp.mpf(1) / 2) / (DEG + 1))
+                                for j in range(DEG + 1)) / (DEG + 1)
+                if c_idx == 0:
+                    s /= 2
+                row.append(float(s))
+        out.append(row)
+    # validation at off-node points, evaluated with double Clenshaw
+    worst = [0.0, 0.0]
+    for t in [-1.0, -0.77, -0.31, 0.13, 0.58, 0.91, 1.0]:
+        ex = rys_exact(n, c + half * mp.mpf(t))
+        for which in (0, 1):
+            for r in range(n):
+                j = which * n + r
+                b1 = b2 = 0.0
+                for m in range(DEG, 0, -1):
+                    b0 = out[m][j] + 2 * t * b1 - b2
+                    b2, b1 = b1, b0
+                v = out[0][j] + t * b1 - b2
+                worst[which] = max(worst[which], abs(v - float(ex[which][r])))
+    return n, seg, out, worst
+
+
+def pick_xcut(n, hh):
+    """Smallest integer x from which the Hermite limit matches to 1e-16 (v) / 5e-17 (w)."""
+    mp.mp.dps = DPS
+    x = 20
+    while True:
+        ok = True
+        for xx in (x, x + 0.5, x + 1, x + 3, x + 10):
+            v, w = rys_exact(n, mp.mpf(xx))
+            va, wa = asymptotic(n, mp.mpf(xx), hh)
+            if max(abs(a - b) for a, b in zip(v, va)) > 1e-16 or max(abs(a - b) for a, b in zip(w, wa)) > 5e-17:
+                ok = False
+                break
+        if ok:
+            return x
+        x += 1
+
+
+def main():
+    mp.mp.dps = DPS
+    herm = {n: hermite_half(n) for n in range(1, NMAX + 1)}
+    xcut = {n: pick_xcut(n, herm[n]) for n in range(1, NMAX + 1)}
+    nseg = {n: int(xcut[n] / SEGLEN) for n in range(1, NMAX + 1)}
+    jobs = [(n, s) for n in range(1, NMAX + 1) for s in range(nseg[n])]
+    with mpr.Pool(min(8, os.cpu_count() or 1)) as pool:
+        res = pool.map(cheb_fit, jobs, chunksize=4)
+    res.sort()
+    off = {}
+    flat = []
+    worst = {n: [0.0, 0.0] for n in range(1, NMAX + 1)}
+    for n in range(1, NMAX + 1):
+        off[n] = len(flat)
+        for (nn, s, rows, w) in res:
+            if nn != n:
+                continue
+            for row in rows:
+                flat.extend(row)
+            worst[n][0] = max(worst[n][0], w[0])
+            worst[n][1] = max(worst[n][1], w[1])
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rys_table.inc")
+    with open(path, "w") as f:
+        f.write("// Generated by gen_rys_table.py -- piecewise Chebyshev Rys nodes/weights.\n")
+        f.write("// Max abs error of double Clenshaw vs %d-digit Golub-Welsch, per n (v, w):\n" % DPS)
+        for n in range(1, NMAX + 1):
+            f.write("//   n=%d xcut=%d nseg=%d  %.2e  %.2e\n" % (n, xcut[n], nseg[n], worst[n][0], worst[n][1]))
+        f.write("// Include after defining RYS_STORAGE (e.g. `static const` on the host,\n")
+        f.write("// `__device__ static const` in device code).\n")
+        f.write("#define RYS_NMAX %d\n#define RYS_DEG %d\n#define RYS_SEGLEN %.17g\n" % (NMAX, DEG, SEGLEN))
+        f.write("RYS_STORAGE int RYS_XCUT[%d] = {0, %s};\n" % (NMAX + 1, ", ".join(str(xcut[n]) for n in range(1, NMAX + 1))))
+        f.write("RYS_STORAGE int RYS_OFF[%d] = {0, %s};\n" % (NMAX + 1, ", ".join(str(off[n]) for n in range(1, NMAX + 1))))
+        hoff = [0]
+        for n in range(1, NMAX + 1):
+            hoff.append(hoff[-1] + n)
+        f.write("RYS_STORAGE int RYS_HOFF[%d] = {%s};\n" % (NMAX + 1, ", ".join(str(o - 0) for o in [0] + hoff[:-1])))
+        f.write("RYS_STORAGE double RYS_HERM_NODE2[%d] = {\n" % hoff[-1])
+        for n in range(1, NMAX + 1):
+            f.write("  " + ", ".join("%.17g" % float(v) for v in herm[n][0]) + ",\n")
+        f.write("};\nRYS_STORAGE double RYS_HERM_W[%d] = {\n" % hoff[-1])
+        for n in range(1, NMAX + 1):
+            f.write("  " + ", ".join("%.17g" % float(v) for v in herm[n][1]) + ",\n")
+        f.write("};\n#define RYS_CHEB_LEN %d\n" % len(flat))
+        f.write("RYS_STORAGE double RYS_CHEB[RYS_CHEB_LEN] = {\n")
+        for i in range(0, len(flat), 4):
+            f.write("  " + ", ".join("%.17g" % v for v in flat[i:i + 4]) + ",\n")
+        f.write("};\n")
+    for n in range(1, NMAX + 1):
+        print("n=%d xcut=%d nseg=%d  max|dv|=%.2e max|dw|=%.2e" % (n, xcut[n], nseg[n], worst[n][0], worst[n][1]))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

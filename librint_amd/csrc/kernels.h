@@ -1,0 +1,47 @@
+// Device-side data layout shared by the host context and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace librint {
+
+struct PrimPair;
+
+// Basis data resident in HBM for the lifetime of a context.
+//   pair_info[p] = {shell a, shell b, first primitive pair, primitive-pair count}
+//   pair_rb[p]   = {Rb, Schwarz bound Q_p}   (Rb = VRR centre)
+//   pair_rab[p]  = {Rb - Ra, 0}
+//   pp[...]      = primitive-pair records (eri_core.h)
+//   ao_off[s]    = first cartesian AO of shell s
+struct DevBasis {
+    const int4 *pair_info;
+    const double4 *pair_rb;
+    const double4 *pair_rab;
+    const PrimPair *pp;
+    const int *ao_off;
+};
+
+// One wavefront of the Fock kernel: bra pair x kets [ket0, ket0 + nket) of
+// the pair list (all kets of one bin).
+struct JKTask {
+    int bra;
+    int ket0;
+    int nket;
+    int pad;
+};
+
+// One thread of the list kernel.
+struct ListTask {
+    int bra;
+    int ket;
+    long long out;
+};
+
+bool class_supported(int la, int lb, int lc, int ld);
+hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, const DevBasis &B,
+                     const double *D, double *M, int nao, hipStream_t st);
+hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
+                       double *out, int mode, int nao, hipStream_t st);
+hipError_t launch_symmetrize(const double *M, double *G, int nao, hipStream_t st);
+
+} // namespace librint

@@ -1,0 +1,169 @@
+"""HIP path vs the oracle through the C ABI (run on an MI355X)."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ref_mol, ref_truth
+from oracle import refcpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _rand_sym(n, seed):
+    rng = np.random.default_rng(seed)
+    P = rng.normal(size=(n, n))
+    return 0.5 * (P + P.T)
+
+
+def test_int2e_tensor_h2o_file_geometry(gpu):
+    from librint_amd import scf
+    _say("int2e h2o/sto3g file geometry")
+    m = ref_mol("h2o", "sto3g", 10)
+    R = scf.int2e(m)
+    assert np.abs(R - refcpu.int2e(m)).max() < 1e-12
+
+
+def test_int2e_tensor_h2o_baseline_geometry(gpu):
+    from librint_amd import mol, scf
+    _say("int2e h2o/sto3g O(0,0,0) H(+-0.757,0.587,0)")
+    m = mol.h2o_sto3g()
+    R = scf.int2e(m)
+    assert np.abs(R - refcpu.int2e(m)).max() < 1e-12
+    for perm in [(1, 0, 2, 3), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.abs(R - R.transpose(perm)).max() == 0.0
+
+
+def test_shell_blocks_alkane_sampled(gpu):
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("shell blocks C16H34/6-31G sampled")
+    m = mol.c16h34_631g()
+    ctx = FockContext(m)
+    rng = np.random.default_rng(5)
+    qs = [tuple(int(v) for v in rng.integers(0, m.nbas, 4)) for _ in range(300)]
+    # include near-diagonal and same-shell quartets
+    qs += [(s, s, s, s) for s in range(0, m.nbas, 7)] + [(3, 4, 3, 4), (10, 2, 2, 10), (0, 1, 2, 3)]
+    got = ctx.shells(qs)
+    worst = 0.0
+    for q, blk in zip(qs, got):
+        worst = max(worst, np.abs(blk - refcpu.int2e_shell(m, q)).max())
+    assert worst < 1e-12
+
+
+@pytest.mark.parametrize("builder", ["h2o_file", "h2o", "c2h6", "c3h8"])
+def test_fock_vs_oracle_full_loop(gpu, builder):
+    from librint_amd import mol, scf
+    _say("fock vs oracle", builder)
+    m = {"h2o_file": lambda: ref_mol("h2o", "sto3g", 10), "h2o": mol.h2o_sto3g,
+         "c2h6": lambda: mol.alkane_631g(2), "c3h8": lambda: mol.alkane_631g(3)}[builder]()
+    n = m.nao()
+    for seed in (1, 2):
+        P = _rand_sym(n, seed)
+        G = scf.fock(m, P, 0.0)
+        Gr = refcpu.fock2e(m, P)
+        assert np.abs(G - Gr).max() < 1e-11 * max(1.0, np.abs(Gr).max())
+
+
+def test_fock_c16h34_block_density(gpu):
+    """Full-size check: a density supported on two shells; the oracle visits
+    only the quartets that can touch it (reference loop restricted)."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("fock C16H34 block density")
+    m = mol.c16h34_631g()
+    ctx = FockContext(m, schwarz_tol=0.0)
+    n = m.nao()
+    bas = m._bas
+    ao = np.concatenate([[0], np.cumsum([(b[1] + 1) * (b[1] + 2) // 2 for b in bas])])
+    S = [40, 43]  # two shells on neighbouring carbons (s and p)
+    P = np.zeros((n, n))
+    rng = np.random.default_rng(9)
+    idx = np.concatenate([np.arange(ao[s], ao[s + 1]) for s in S])
+    B = rng.normal(size=(len(idx), len(idx)))
+    P[np.ix_(idx, idx)] = B + B.T
+    G = ctx.fock(P)
+    nb = m.nbas
+    qs = set()
+    for i in range(nb):
+        for j in range(nb):
+            for a in S:
+                for b in S:
+                    qs.add((i, j, a, b))  # J: P[l,k] with k,l in S
+                    qs.add((i, a, b, j))  # K: P[j,k] with j,k in S
+    qs = np.array(sorted(qs), dtype=np.int32)
+    Gr = refcpu.fock2e_sample(m, P, qs)
+    assert np.abs(G - Gr).max() < 1e-11 * np.abs(Gr).max()
+    assert np.abs(G - G.T).max() < 1e-13 * np.abs(G).max()
+
+
+def test_fock_linearity_and_screening(gpu):
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("fock linearity C16H34")
+    m = mol.c16h34_631g()
+    ctx = FockContext(m, schwarz_tol=1e-12)
+    n = m.nao()
+    P1, P2 = _rand_sym(n, 3), _rand_sym(n, 4)
+    G1, G2, G12 = ctx.fock(P1), ctx.fock(P2), ctx.fock(2.0 * P1 - 0.5 * P2)
+    assert np.abs(G12 - (2.0 * G1 - 0.5 * G2)).max() < 1e-11 * np.abs(G12).max()
+    st = ctx.stats()
+    ctx.set_schwarz(0.0)
+    G0 = ctx.fock(P1)
+    assert ctx.stats()["quartets"] > st["quartets"]
+    # Schwarz at 1e-12 on a density with |P| ~ 1: error bounded by tau * sum|P| per element
+    assert np.abs(G0 - G1).max() < 1e-12 * np.abs(P1).sum()
+
+
+@pytest.mark.parametrize("name,nelec", [("h2", 2), ("h2o", 10)])
+def test_scf_c_truth_energy(gpu, name, nelec):
+    from librint_amd import scf
+    _say("scf_c truth", name)
+    m = ref_mol(name, "sto3g", nelec)
+    E = scf.scf(m, 200, 1e-6)
+    assert abs(E - float(ref_truth(name, "sto3g_E"))) < 1e-10
+
+
+def test_density_energy_and_baseline_geometry(gpu):
+    from librint_amd import mol, scf
+    _say("density/energy h2o baseline geometry")
+    m = mol.h2o_sto3g()
+    P = scf.density(m, 200, 1e-10)
+    rc, Pr, it = refcpu.density(m, 200, 1e-10)
+    assert rc == 0
+    assert np.abs(P - Pr).max() < 1e-8
+    E = scf.energy(m, P)
+    assert abs(E - refcpu.energy(m, Pr)) < 1e-10
+    assert abs(E - scf.scf(m, 200, 1e-10)) < 1e-10
+
+
+def test_failures_are_reported(gpu):
+    from librint_amd import scf
+    _say("failure paths")
+    m = ref_mol("h2o", "sto3g", 10)
+    with pytest.raises(RuntimeError):
+        scf.density(m, 1, 1e-12)  # cannot converge in one cycle
+    with pytest.raises(RuntimeError):
+        scf.scf(m, 1, 1e-12)
+    with pytest.raises(RuntimeError):
+        scf.int2e(m, "sph")
+
+
+def test_fock_device_buffers(gpu):
+    import torch
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("fock device buffers")
+    m = mol.alkane_631g(4)
+    ctx = FockContext(m, schwarz_tol=1e-14)
+    P = _rand_sym(m.nao(), 8)
+    Pd = torch.from_numpy(P).to(gpu)
+    Gd = torch.empty_like(Pd)
+    ctx.fock_device(Pd, Gd)
+    torch.cuda.synchronize()
+    assert np.abs(Gd.cpu().numpy() - ctx.fock(P)).max() == 0.0

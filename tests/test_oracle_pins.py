@@ -1,0 +1,145 @@
+"""The oracle (oracle/refcpu.c) against every pin the reference holds for
+this path, plus mathematical pins for what those leave open."""
+import itertools
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLD, ref_mol, ref_truth
+from oracle import refcpu
+import textbook_eri as tb
+
+G = json.load(open(os.path.join(GOLD, "rys_mpmath.json")))
+
+
+@pytest.mark.parametrize("name,nelec,n", [("h2", 2, 2), ("h2o", 10, 7)])
+@pytest.mark.parametrize("typ,key", [(0, "S"), (1, "T"), (2, "V")])
+def test_int1e_truth_sto3g(name, nelec, n, typ, key):
+    m = ref_mol(name, "sto3g", nelec)
+    M = refcpu.int1e(m, typ)
+    ref = ref_truth(name, "sto3g_" + key).reshape(n, n)
+    assert np.abs(M - ref).max() < 1e-12  # reference checks/overlap.rs:20 epsilon
+
+
+@pytest.mark.parametrize("typ,key", [(0, "S"), (1, "T"), (2, "V")])
+def test_int1e_truth_c6h6_631g_dshells(typ, key):
+    m = ref_mol("c6h6", "631g", 42)
+    M = refcpu.int1e(m, typ)
+    ref = ref_truth("c6h6", "631g_" + key).reshape(102, 102)
+    assert np.abs(M - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("name,nelec", [("h2", 2), ("h2o", 10)])
+def test_scf_energy_truth(name, nelec):
+    # reference checks/energy.rs:182-196: imax 200, conv 1e-6 on |dP|/2
+    m = ref_mol(name, "sto3g", nelec)
+    E = refcpu.scf(m, 200, 1e-6)
+    Eref = float(ref_truth(name, "sto3g_E"))
+    assert abs(E - Eref) < 1e-10
+
+
+def test_boys_vs_mpmath():
+    for e in G["boys"]:
+        F = refcpu.boys(18, e["x"])
+        ref = np.array(e["F"])
+        assert np.all(np.abs(F - ref) <= 4e-16 * np.abs(ref) + 1e-300), e["x"]
+
+
+def test_rys_vs_mpmath():
+    for e in G["rys"]:
+        v, w = refcpu.rys_nodes(e["n"], e["x"])
+        assert np.abs(v - e["v"]).max() < 5e-16, (e["n"], e["x"])
+        assert np.abs(w - e["w"]).max() < 5e-16, (e["n"], e["x"])
+
+
+@pytest.mark.parametrize("n", [1, 3, 5, 9])
+def test_rys_moments(n):
+    # sum_r w_r v_r^k = F_k(x) for k < 2n (Gauss rule exactness)
+    for x in [0.0, 0.3, 2.0, 9.0, 27.0, 44.0, 70.0, 120.0]:
+        v, w = refcpu.rys_nodes(n, x)
+        F = refcpu.boys(2 * n - 1, x)
+        for k in range(2 * n):
+            assert abs(np.sum(w * v ** k) - F[k]) <= 1e-15 * F[0] + 1e-14 * F[k], (n, x, k)
+
+
+def test_rys_large_x_hermite_limit():
+    # t_r^2 -> h_r / x and w_r -> H_r / sqrt(x) with the Gauss-Hermite rule
+    for n in (1, 2, 4):
+        s, h = np.polynomial.hermite.hermgauss(2 * n)
+        pos = s > 0
+        x = 400.0
+        v, w = refcpu.rys_nodes(n, x)
+        assert np.allclose(v, np.sort(s[pos] ** 2) / x, rtol=1e-13, atol=0)
+        assert np.allclose(w, h[pos][np.argsort(s[pos] ** 2)] / np.sqrt(x), rtol=1e-13, atol=0)
+
+
+def test_ssss_closed_form():
+    # (ss|ss) for single primitives: 2 pi^{5/2}/(p q sqrt(p+q)) e^{-..} e^{-..} F0(T)
+    from librint_amd import mol
+    atoms = [(1, (0.0, 0.0, 0.0)), (1, (0.0, 0.3, 1.1)), (1, (0.7, -0.2, 2.0)), (1, (1.5, 0.4, -0.3))]
+    basis = {1: [(0, [0.9], [1.0])]}
+    m = mol.build(atoms, basis, normalized=True)
+    c = 1.0 * 0.282094791773878143
+    A = np.array([a[1] for a in atoms])
+    a = 0.9
+    p = q = 2 * a
+    P = (A[0] + A[1]) / 2
+    Q = (A[2] + A[3]) / 2
+    T = p * q / (p + q) * np.sum((P - Q) ** 2)
+    ref = 2 * np.pi ** 2.5 / (p * q * np.sqrt(p + q)) * np.exp(-a / 2 * np.sum((A[0] - A[1]) ** 2)) \
+        * np.exp(-a / 2 * np.sum((A[2] - A[3]) ** 2)) * tb.boys(0, T)[0] * c ** 4
+    got = refcpu.int2e_shell(m, (0, 1, 2, 3))[0, 0, 0, 0]
+    assert abs(got - ref) < 1e-15
+
+
+def test_eri_vs_textbook_h2o_all_quartets():
+    m = ref_mol("h2o", "sto3g", 10)
+    for q in itertools.product(range(m.nbas), repeat=4):
+        assert np.abs(tb.eri_block(m, q) - refcpu.int2e_shell(m, q)).max() < 1e-13, q
+
+
+def test_eri_vs_textbook_d_shells():
+    m = ref_mol("h2o", "def2svp", 10)
+    rng = np.random.default_rng(7)
+    qs = [(5, 5, 5, 5), (5, 3, 5, 0), (11, 5, 2, 5)] + [tuple(int(v) for v in rng.integers(0, m.nbas, 4))
+                                                        for _ in range(10)]
+    for q in qs:
+        assert np.abs(tb.eri_block(m, q) - refcpu.int2e_shell(m, q)).max() < 1e-13, q
+
+
+def test_eri_symmetry_and_schwarz():
+    m = ref_mol("h2o", "sto3g", 10)
+    R = refcpu.int2e(m)
+    for perm in [(1, 0, 2, 3), (0, 1, 3, 2), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.abs(R - R.transpose(perm)).max() < 1e-14
+    n = R.shape[0]
+    M = R.reshape(n * n, n * n)
+    assert np.linalg.eigvalsh(M).min() > -1e-12  # (ij|kl) as a pair matrix is PSD
+    d = np.sqrt(np.abs(np.diag(M)))
+    assert np.all(np.abs(M) <= np.outer(d, d) + 1e-14)
+
+
+def test_fock_direct_matches_tensor():
+    # integral2e_fock (src/scf.rs:316-410) == calc_F - H (src/scf.rs:651-670)
+    m = ref_mol("h2o", "sto3g", 10)
+    rng = np.random.default_rng(3)
+    P = rng.normal(size=(7, 7))
+    P = P + P.T
+    R = refcpu.int2e(m)
+    J = np.einsum("ijkl,lk->ij", R, P)
+    K = np.einsum("ijkl,jk->il", R, P)
+    G = refcpu.fock2e(m, P)
+    assert np.abs(G - (J - 0.5 * K)).max() < 1e-13
+
+
+def test_planted_mistake_is_caught():
+    # a wrong sign in the exchange term must fail the tensor comparison above
+    m = ref_mol("h2o", "sto3g", 10)
+    rng = np.random.default_rng(3)
+    P = rng.normal(size=(7, 7))
+    P = P + P.T
+    R = refcpu.int2e(m)
+    wrong = np.einsum("ijkl,lk->ij", R, P) + 0.5 * np.einsum("ijkl,jk->il", R, P)
+    assert np.abs(refcpu.fock2e(m, P) - wrong).max() > 1e-3

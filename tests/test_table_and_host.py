@@ -1,0 +1,131 @@
+"""Product-side pieces that run without a GPU: the Rys table the kernels use
+(evaluated here in numpy exactly as the device code does), the molecule
+builders, the host one-electron integrals, and the C-ABI export list."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLD, ROOT, ref_mol, ref_truth
+
+G = json.load(open(os.path.join(GOLD, "rys_mpmath.json")))
+INC = os.path.join(ROOT, "librint_amd", "csrc", "rys_table.inc")
+
+
+def _table():
+    txt = open(INC).read()
+
+    def arr(name):
+        m = re.search(name + r"\[[^\]]*\] = \{(.*?)\};", txt, re.S)
+        return np.array([float(x) for x in m.group(1).replace("\n", " ").split(",") if x.strip()])
+
+    deg = int(re.search(r"#define RYS_DEG (\d+)", txt).group(1))
+    return (arr("RYS_CHEB"), arr("RYS_OFF").astype(int), arr("RYS_XCUT").astype(int),
+            arr("RYS_HOFF").astype(int), arr("RYS_HERM_NODE2"), arr("RYS_HERM_W"), deg)
+
+
+TAB = _table()
+
+
+def table_nodes(n, x):
+    cheb, off, xcut, hoff, h2, hw, deg = TAB
+    if x >= xcut[n]:
+        return h2[hoff[n]:hoff[n] + n] / x, hw[hoff[n]:hoff[n] + n] / np.sqrt(x)
+    k = int(x)
+    s = 2 * (x - k) - 1
+    nv = 2 * n
+    c = cheb[off[n] + k * (deg + 1) * nv: off[n] + (k + 1) * (deg + 1) * nv].reshape(deg + 1, nv)
+    b1 = np.zeros(nv)
+    b2 = np.zeros(nv)
+    for m in range(deg, 0, -1):
+        b0 = 2 * s * b1 + c[m] - b2
+        b2, b1 = b1, b0
+    val = s * b1 + c[0] - b2
+    return val[:n], val[n:]
+
+
+def test_table_vs_mpmath():
+    for e in G["rys"]:
+        v, w = table_nodes(e["n"], e["x"])
+        assert np.abs(v - e["v"]).max() < 3e-16, (e["n"], e["x"])
+        assert np.abs(w - e["w"]).max() < 3e-16, (e["n"], e["x"])
+
+
+def test_table_vs_oracle_dense():
+    from oracle import refcpu
+    rng = np.random.default_rng(11)
+    for n in range(1, 10):
+        for x in np.concatenate([rng.uniform(0, 90, 60), [0.0, 1e-12, float(TAB[2][n]) - 1e-9, float(TAB[2][n])]]):
+            v, w = table_nodes(n, x)
+            vr, wr = refcpu.rys_nodes(n, float(x))
+            assert np.abs(v - vr).max() < 4e-16 and np.abs(w - wr).max() < 4e-16, (n, x)
+
+
+def test_molecule_builders():
+    from librint_amd import mol
+    m = mol.c16h34_631g()
+    assert (m.nbas, m.nao(), m.nelectron) == (148, 212, 130)
+    C, H = mol.alkane_geometry(16)
+    d = np.linalg.norm(C[1:] - C[:-1], axis=1)
+    assert np.allclose(d, 1.54)
+    dch = np.array([min(np.linalg.norm(h - c) for c in C) for h in H])
+    assert np.allclose(dch, 1.09) and len(H) == 34
+    ang = [np.degrees(np.arccos(np.dot(C[i - 1] - C[i], C[i + 1] - C[i]) / 1.54 ** 2)) for i in range(1, 15)]
+    assert np.allclose(ang, 109.4712206, atol=1e-6)
+    w = mol.water_cluster(32)
+    assert (w.nbas, w.nao(), w.nelectron, w.max_l()) == (512, 992, 320, 2)
+    O = mol.BOHR * w.coords()[::3]
+    dd = np.linalg.norm(O[:, None] - O[None], axis=-1) + np.eye(32) * 10
+    assert dd.min() >= 2.6 and O.min() >= 0 and O.max() <= 10 + 1.0
+
+
+def test_normalisation_matches_reference_env():
+    from librint_amd import mol
+    # raw STO-3G H coefficients -> the normalised values stored in the reference files
+    raw = [0.15432897, 0.53532814, 0.44463454]
+    got = mol.normalize_contraction(0, [3.42525091, 0.62391373, 0.1688554], raw)
+    assert np.allclose(got, mol.STO3G[1][0][2], rtol=1e-8)
+    # normalised contracted functions have unit self-overlap (checked via the reference file's env)
+    for (l, e, c) in mol.B631G[6]:
+        again = mol.normalize_contraction(l, e, np.array(c) / np.array([mol.gto_norm(l, a) for a in e]))
+        assert np.allclose(again, c, rtol=1e-10)
+
+
+def test_header_exports_match_library():
+    from librint_amd import _bindings
+    hdr = open(os.path.join(ROOT, "include", "librint.h")).read()
+    names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)))
+    names -= {"extern", "if", "defined"}
+    declared = {n for n in names if n.endswith("_c") or n.startswith("librint_") or n in ("dS_u", "free_c")}
+    assert declared == set(_bindings.EXPORTED)
+    for n in declared:
+        assert hasattr(_bindings.library, n), n
+
+
+@pytest.mark.parametrize("name,nelec,n", [("h2", 2, 2), ("h2o", 10, 7)])
+@pytest.mark.parametrize("typ,key", [("ovlp", "S"), ("kin", "T"), ("nuc", "V")])
+def test_library_int1e_truth(name, nelec, n, typ, key):
+    from librint_amd import scf
+    m = ref_mol(name, "sto3g", nelec)
+    assert np.abs(scf.int1e(m, typ) - ref_truth(name, "sto3g_" + key).reshape(n, n)).max() < 1e-12
+
+
+@pytest.mark.parametrize("typ,key", [("ovlp", "S"), ("kin", "T"), ("nuc", "V")])
+def test_library_int1e_truth_c6h6(typ, key):
+    from librint_amd import scf
+    m = ref_mol("c6h6", "631g", 42)
+    assert np.abs(scf.int1e(m, typ) - ref_truth("c6h6", "631g_" + key).reshape(102, 102)).max() < 1e-12
+
+
+def test_library_rejects_bad_codes():
+    from librint_amd import scf, utils
+    m = ref_mol("h2o", "sto3g", 10)
+    with pytest.raises(RuntimeError):
+        scf.int1e(m, "ovlp", "sph")
+    atm, bas, env, _ = utils.prep(m)
+    from librint_amd._bindings import library
+    assert not library.int1e_c(*utils.mol_args(atm, bas, env), 7, 0)
+    assert not library.int1e_c(*utils.mol_args(atm, bas, env), 0, 9)
+    assert not library.grad_c(*utils.mol_args(atm, bas, env), None, 0)

@@ -1,0 +1,130 @@
+"""Independent textbook ERI (McMurchie-Davidson, Helgaker/Jorgensen/Olsen,
+"Molecular Electronic-Structure Theory", sec. 9.5 and 9.9) in numpy/scipy.
+
+Used only to pin the oracle's Rys-quadrature ERIs: a different algorithm
+(Hermite Gaussians + Boys function from scipy's incomplete gamma), so a
+dropped term, wrong sign or transposed index in either one fails the test.
+"""
+import math
+
+import numpy as np
+from scipy.special import gammainc, gamma
+
+FAC_SP = {0: 0.282094791773878143, 1: 0.488602511902919921}
+
+
+def boys(nmax, x):
+    out = np.empty(nmax + 1)
+    for m in range(nmax + 1):
+        if x < 1e-8:
+            out[m] = 1.0 / (2 * m + 1) - x / (2 * m + 3)
+        else:
+            a = m + 0.5
+            out[m] = gamma(a) * gammainc(a, x) / (2.0 * x ** a)
+    return out
+
+
+def hermite_E(i, j, a, b, XAB):
+    """E[i][j][t] for one axis, including exp(-mu X_AB^2)."""
+    p = a + b
+    mu = a * b / p
+    XPA = -b / p * XAB
+    XPB = a / p * XAB
+    E = np.zeros((i + 1, j + 1, i + j + 2))
+    E[0, 0, 0] = math.exp(-mu * XAB * XAB)
+    for ii in range(i + 1):
+        for jj in range(j + 1):
+            if ii == 0 and jj == 0:
+                continue
+            for t in range(ii + jj + 1):
+                if ii > 0:
+                    v = XPA * E[ii - 1, jj, t] + (t + 1) * E[ii - 1, jj, t + 1]
+                    if t > 0:
+                        v += E[ii - 1, jj, t - 1] / (2 * p)
+                else:
+                    v = XPB * E[ii, jj - 1, t] + (t + 1) * E[ii, jj - 1, t + 1]
+                    if t > 0:
+                        v += E[ii, jj - 1, t - 1] / (2 * p)
+                E[ii, jj, t] = v
+    return E
+
+
+def hermite_R(L, alpha, PQ):
+    """R[t][u][v] = R^0_{tuv}(alpha, PQ) for t+u+v <= L."""
+    X, Y, Z = PQ
+    F = boys(L, alpha * (X * X + Y * Y + Z * Z))
+    Rn = np.zeros((L + 1, L + 1, L + 1, L + 1))
+    for n in range(L + 1):
+        Rn[n, 0, 0, 0] = (-2 * alpha) ** n * F[n]
+    for tot in range(1, L + 1):
+        for n in range(L - tot + 1):
+            for t in range(tot + 1):
+                for u in range(tot - t + 1):
+                    v = tot - t - u
+                    if t > 0:
+                        val = X * Rn[n + 1, t - 1, u, v] + ((t - 1) * Rn[n + 1, t - 2, u, v] if t > 1 else 0)
+                    elif u > 0:
+                        val = Y * Rn[n + 1, t, u - 1, v] + ((u - 1) * Rn[n + 1, t, u - 2, v] if u > 1 else 0)
+                    else:
+                        val = Z * Rn[n + 1, t, u, v - 1] + ((v - 1) * Rn[n + 1, t, u, v - 2] if v > 1 else 0)
+                    Rn[n, t, u, v] = val
+    return Rn[0]
+
+
+def cart(l):
+    out = []
+    for lx in range(l, -1, -1):
+        for ly in range(l - lx, -1, -1):
+            out.append((lx, ly, l - lx - ly))
+    return out
+
+
+def shell(mol, s):
+    b = mol._bas[s]
+    a = mol._atm[b[0]]
+    R = mol._env[a[1]:a[1] + 3]
+    l, nprim = int(b[1]), int(b[2])
+    exps = mol._env[b[5]:b[5] + nprim]
+    coefs = mol._env[b[6]:b[6] + nprim] * FAC_SP.get(l, 1.0)
+    return l, R, exps, coefs
+
+
+def eri_block(mol, shls):
+    """(ij|kl) block, shape (nfi, nfj, nfk, nfl), segmented shells."""
+    sh = [shell(mol, s) for s in shls]
+    (la, A, ea, ca), (lb, B, eb, cb), (lc, C, ec, cc), (ld, D, ed, cd) = sh
+    fa, fb, fc, fd = cart(la), cart(lb), cart(lc), cart(ld)
+    out = np.zeros((len(fa), len(fb), len(fc), len(fd)))
+    L = la + lb + lc + ld
+    for a, c1 in zip(ea, ca):
+        for b, c2 in zip(eb, cb):
+            p = a + b
+            P = (a * A + b * B) / p
+            Eab = [hermite_E(la, lb, a, b, A[x] - B[x]) for x in range(3)]
+            for c, c3 in zip(ec, cc):
+                for d, c4 in zip(ed, cd):
+                    q = c + d
+                    Q = (c * C + d * D) / q
+                    Ecd = [hermite_E(lc, ld, c, d, C[x] - D[x]) for x in range(3)]
+                    alpha = p * q / (p + q)
+                    R = hermite_R(L, alpha, P - Q)
+                    pref = 2 * math.pi ** 2.5 / (p * q * math.sqrt(p + q)) * c1 * c2 * c3 * c4
+                    for ia, (ax, ay, az) in enumerate(fa):
+                        for ib, (bx, by, bz) in enumerate(fb):
+                            for ic, (cx, cy, cz) in enumerate(fc):
+                                for id_, (dx, dy, dz) in enumerate(fd):
+                                    s = 0.0
+                                    for t in range(ax + bx + 1):
+                                        for u in range(ay + by + 1):
+                                            for v in range(az + bz + 1):
+                                                e1 = Eab[0][ax, bx, t] * Eab[1][ay, by, u] * Eab[2][az, bz, v]
+                                                if e1 == 0.0:
+                                                    continue
+                                                for tau in range(cx + dx + 1):
+                                                    for nu in range(cy + dy + 1):
+                                                        for phi in range(cz + dz + 1):
+                                                            e2 = Ecd[0][cx, dx, tau] * Ecd[1][cy, dy, nu] * Ecd[2][cz, dz, phi]
+                                                            sgn = -1.0 if (tau + nu + phi) % 2 else 1.0
+                                                            s += e1 * sgn * e2 * R[t + tau, u + nu, v + phi]
+                                    out[ia, ib, ic, id_] += pref * s
+    return out
