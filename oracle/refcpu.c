@@ -20,7 +20,7 @@
  * reference):
  *   - Boys: power series below a turnover, erf + upward recursion above,
  *     in long double.
- *   - Rys rule: discretised Stieltjes procedure on a 120-point Gauss-Legendre
+ *   - Rys rule: discretised Stieltjes procedure on a 56-point Gauss-Legendre
  *     discretisation of exp(-x t^2) dt, then a Jacobi eigen-solve of the
  *     n x n Jacobi matrix, all in long double (Gautschi, "Orthogonal
  *     Polynomials: Computation and Approximation", 2004, sec. 2.2.3).
@@ -74,7 +74,7 @@ void rc_boys(int mmax, double x, double *out)
 }
 
 /* ------------------------------------------------------------ Rys rule -- */
-#define NGL 120
+#define NGL 56
 static ld gl_x[NGL], gl_w[NGL];
 static int gl_ready = 0;
 
@@ -117,10 +117,12 @@ static void jacobi_ld(int n, ld *A, ld *V)
     for (int i = 0; i < n; i++)
         for (int j = 0; j < n; j++) V[i * n + j] = (i == j);
     for (int sweep = 0; sweep < 100; sweep++) {
-        ld off = 0;
-        for (int i = 0; i < n; i++)
+        ld off = 0, dia = 0;
+        for (int i = 0; i < n; i++) {
+            dia += A[i * n + i] * A[i * n + i];
             for (int j = i + 1; j < n; j++) off += A[i * n + j] * A[i * n + j];
-        if (off < 1e-70L) break;
+        }
+        if (off <= 1e-40L * dia) break;
         for (int p = 0; p < n; p++)
             for (int q = p + 1; q < n; q++) {
                 ld apq = A[p * n + q];
