@@ -183,6 +183,267 @@ __global__ __launch_bounds__(64) void jk_kernel(const JKTask *__restrict__ tasks
     }
 }
 
+// ------------------------------------------------------ team J/K kernel --
+// Classes whose contracted block does not fit in one lane's registers
+// (NF > 108, e.g. (dd|dd) with 1296 integrals and 5 roots): one wavefront
+// works on one quartet at a time, looping over the kets of its task.
+// Per primitive quartet, lanes c < 3*NROOTS build one (root, axis) chain of
+// the VRR/HRR in LDS; then every lane accumulates its share of the NF
+// outputs (f = lane + 64k) from the LDS chains.  The J/K blocks of a quartet
+// are reduced in LDS (ds_add_f64) and flushed with one global atomic per
+// element; the bra-bra block stays in LDS for the whole task.
+__device__ __constant__ signed char CARTP[3][6][3] = {
+    {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
+
+__device__ __forceinline__ double sel3(int ax, double x, double y, double z) { return ax == 0 ? x : (ax == 1 ? y : z); }
+
+template <int LA, int LB, int LC, int LD>
+struct TeamShape {
+    using C = Class<LA, LB, LC, LD>;
+    static constexpr int NCH = 3 * C::NROOTS;
+    static constexpr int NPL = (C::NF + 63) / 64;
+    static constexpr int NAB = C::NFA * C::NFB, NCD = C::NFC * C::NFD;
+    static constexpr int NAC = C::NFA * C::NFC, NBC = C::NFB * C::NFC;
+    static constexpr int NAD = C::NFA * C::NFD, NBD = C::NFB * C::NFD;
+    static constexpr int NKET = NCD + NAC + NBC + NAD + NBD;
+};
+
+// one primitive quartet's chains into LDS (lanes c < NCH)
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void team_chain(double *g, int c, const PrimPair &p, const PrimPair &q, const double rbd[3],
+                                           const double rab[3], const double rcd[3])
+{
+    using C = Class<LA, LB, LC, LD>;
+    constexpr double TWO_PI_52 = 34.98683665524972497;
+    const double aij = p.e, akl = q.e;
+    const double a1 = aij * akl;
+    const double rs = 1.0 / (aij + akl);
+    const double a0 = a1 * rs;
+    const double pqx = p.pbx - q.pbx + rbd[0], pqy = p.pby - q.pby + rbd[1], pqz = p.pbz - q.pbz + rbd[2];
+    const double rr = pqx * pqx + pqy * pqy + pqz * pqz;
+    const double fac1 = TWO_PI_52 * p.K * q.K * sqrt(rs) / a1;
+    double v[C::NROOTS], w[C::NROOTS];
+    rys_nodes<C::NROOTS>(a0 * rr, v, w);
+    const int r = c / 3, ax = c - 3 * (c / 3);
+    double t2 = v[0], wr = w[0];
+#pragma unroll
+    for (int k = 1; k < C::NROOTS; k++)
+        if (r == k) {
+            t2 = v[k];
+            wr = w[k];
+        }
+    const double tr = t2 * rs;
+    const double b00 = 0.5 * tr;
+    const double b10 = 0.5 / aij * (1.0 - akl * tr);
+    const double b01 = 0.5 / akl * (1.0 - aij * tr);
+    const double pq = sel3(ax, pqx, pqy, pqz);
+    const double cn = sel3(ax, p.pbx, p.pby, p.pbz) - akl * tr * pq;
+    const double cm = sel3(ax, q.pbx, q.pby, q.pbz) + aij * tr * pq;
+    double *gc = g + c * C::GS;
+    gc[0] = (ax == 2) ? wr * fac1 : 1.0;
+    vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gc, cn, cm, b10, b01, b00);
+    if constexpr (LA > 0 || LC > 0)
+        hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gc, sel3(ax, rab[0], rab[1], rab[2]),
+                                                                   sel3(ax, rcd[0], rcd[1], rcd[2]));
+}
+
+// contracted block of one quartet, distributed: acc[k] holds f = lane + 64k
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void team_contracted(double *acc, double *g, const int *pk, const DevBasis &B,
+                                                const PairView &bra, const PairView &ket, int lane)
+{
+    using C = Class<LA, LB, LC, LD>;
+    using S = TeamShape<LA, LB, LC, LD>;
+#pragma unroll
+    for (int k = 0; k < S::NPL; k++) acc[k] = 0.0;
+    const double rbd[3] = {bra.rb[0] - ket.rb[0], bra.rb[1] - ket.rb[1], bra.rb[2] - ket.rb[2]};
+    for (int ib = 0; ib < bra.npp; ib++) {
+        const PrimPair p = B.pp[bra.off + ib];
+        for (int ik = 0; ik < ket.npp; ik++) {
+            const PrimPair q = B.pp[ket.off + ik];
+            if (lane < S::NCH) team_chain<LA, LB, LC, LD>(g, lane, p, q, rbd, bra.rab, ket.rab);
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < S::NPL; k++) {
+                if (lane + 64 * k >= C::NF) continue;
+                const int ix = pk[k] & 255, iy = (pk[k] >> 8) & 255, iz = pk[k] >> 16;
+                double a = acc[k];
+#pragma unroll
+                for (int r = 0; r < C::NROOTS; r++)
+                    a += g[(3 * r) * C::GS + ix] * g[(3 * r + 1) * C::GS + iy] * g[(3 * r + 2) * C::GS + iz];
+                acc[k] = a;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// packed g offsets (ix | iy << 8 | iz << 16, all < GS <= 225) of the outputs
+// f = lane + 64k a lane owns
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void team_indices(int lane, int *pk)
+{
+    using C = Class<LA, LB, LC, LD>;
+    using S = TeamShape<LA, LB, LC, LD>;
+    static_assert(C::GS <= 256, "packed offsets");
+#pragma unroll
+    for (int k = 0; k < S::NPL; k++) {
+        const int f = lane + 64 * k;
+        const int ff = f < C::NF ? f : 0;
+        const int fa = ff % C::NFA, t1 = ff / C::NFA;
+        const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
+        const int fc = t2 % C::NFC, fd = t2 / C::NFC;
+        const int ix = CARTP[LB][fb][0] * C::DJ + CARTP[LD][fd][0] * C::DL + CARTP[LC][fc][0] * C::DK + CARTP[LA][fa][0];
+        const int iy = CARTP[LB][fb][1] * C::DJ + CARTP[LD][fd][1] * C::DL + CARTP[LC][fc][1] * C::DK + CARTP[LA][fa][1];
+        const int iz = CARTP[LB][fb][2] * C::DJ + CARTP[LD][fd][2] * C::DL + CARTP[LC][fc][2] * C::DK + CARTP[LA][fa][2];
+        pk[k] = ix | (iy << 8) | (iz << 16);
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void jk_team_kernel(const JKTask *__restrict__ tasks, DevBasis B,
+                                                     const double *__restrict__ D, double *__restrict__ M, int nao)
+{
+    using C = Class<LA, LB, LC, LD>;
+    using S = TeamShape<LA, LB, LC, LD>;
+    __shared__ double g[S::NCH * C::GS];
+    __shared__ double mab[S::NAB];
+    __shared__ double mk[S::NKET];
+    __shared__ double accs[S::NPL * 64];  // contracted outputs, read back one at a time for J/K
+    const JKTask t = tasks[blockIdx.x];
+    const int lane = threadIdx.x;
+    const PairView bra = load_pair(B, t.bra);
+    int pk[S::NPL];
+    team_indices<LA, LB, LC, LD>(lane, pk);
+    for (int e = lane; e < S::NAB; e += 64) mab[e] = 0.0;
+    const size_t n = (size_t)nao;
+    double *mcd = mk, *mac = mk + S::NCD, *mbc = mac + S::NAC, *mad = mbc + S::NBC, *mbd = mad + S::NAD;
+    for (int kk = 0; kk < t.nket; kk++) {
+        const int ketp = t.ket0 + kk;
+        const PairView ket = load_pair(B, ketp);
+        double acc[S::NPL];
+        team_contracted<LA, LB, LC, LD>(acc, g, pk, B, bra, ket, lane);
+        double s = 1.0;
+        if (bra.sa == bra.sb) s *= 0.5;
+        if (ket.sa == ket.sb) s *= 0.5;
+        if (t.bra == ketp) s *= 0.5;
+        for (int e = lane; e < S::NKET; e += 64) mk[e] = 0.0;
+#pragma unroll
+        for (int k = 0; k < S::NPL; k++) accs[k * 64 + lane] = acc[k];
+        __syncthreads();
+#pragma unroll 1
+        for (int k = 0; k < S::NPL; k++) {
+            const int f = lane + 64 * k;
+            if (f >= C::NF) continue;
+            const int fa = f % C::NFA, t1 = f / C::NFA;
+            const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
+            const int fc = t2 % C::NFC, fd = t2 / C::NFC;
+            const size_t a = bra.aoa + fa, b = bra.aob + fb, c = ket.aoa + fc, d = ket.aob + fd;
+            const double v = s * accs[k * 64 + lane];
+            atomicAdd(&mab[fa * C::NFB + fb], 2.0 * v * D[c * n + d]);
+            atomicAdd(&mcd[fc * C::NFD + fd], 2.0 * v * D[a * n + b]);
+            atomicAdd(&mac[fa * C::NFC + fc], -0.5 * v * D[b * n + d]);
+            atomicAdd(&mbc[fb * C::NFC + fc], -0.5 * v * D[a * n + d]);
+            atomicAdd(&mad[fa * C::NFD + fd], -0.5 * v * D[b * n + c]);
+            atomicAdd(&mbd[fb * C::NFD + fd], -0.5 * v * D[a * n + c]);
+        }
+        __syncthreads();
+        for (int e = lane; e < S::NKET; e += 64) {
+            size_t row, col;
+            if (e < S::NCD) {
+                row = ket.aoa + e / C::NFD;
+                col = ket.aob + e % C::NFD;
+            } else if (e < S::NCD + S::NAC) {
+                const int x = e - S::NCD;
+                row = bra.aoa + x / C::NFC;
+                col = ket.aoa + x % C::NFC;
+            } else if (e < S::NCD + S::NAC + S::NBC) {
+                const int x = e - S::NCD - S::NAC;
+                row = bra.aob + x / C::NFC;
+                col = ket.aoa + x % C::NFC;
+            } else if (e < S::NCD + S::NAC + S::NBC + S::NAD) {
+                const int x = e - S::NCD - S::NAC - S::NBC;
+                row = bra.aoa + x / C::NFD;
+                col = ket.aob + x % C::NFD;
+            } else {
+                const int x = e - S::NCD - S::NAC - S::NBC - S::NAD;
+                row = bra.aob + x / C::NFD;
+                col = ket.aob + x % C::NFD;
+            }
+            atomicAdd(&M[row * n + col], mk[e]);
+        }
+        __syncthreads();
+    }
+    for (int e = lane; e < S::NAB; e += 64) atomicAdd(&M[(bra.aoa + e / C::NFB) * n + bra.aob + e % C::NFB], mab[e]);
+}
+
+// list variant for the team classes: one wavefront per list task
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(64) void eri_list_team_kernel(const ListTask *__restrict__ tasks, int ntask, DevBasis B,
+                                                           double *__restrict__ out, int mode, int nao)
+{
+    using C = Class<LA, LB, LC, LD>;
+    using S = TeamShape<LA, LB, LC, LD>;
+    __shared__ double g[S::NCH * C::GS];
+    __shared__ double red[64];
+    if ((int)blockIdx.x >= ntask) return;
+    const ListTask t = tasks[blockIdx.x];
+    const int lane = threadIdx.x;
+    int pk[S::NPL];
+    team_indices<LA, LB, LC, LD>(lane, pk);
+    const PairView bra = load_pair(B, t.bra);
+    const PairView ket = load_pair(B, t.ket);
+    double acc[S::NPL];
+    team_contracted<LA, LB, LC, LD>(acc, g, pk, B, bra, ket, lane);
+    const size_t n = (size_t)nao;
+    if (mode == 0) {
+#pragma unroll
+        for (int k = 0; k < S::NPL; k++)
+            if (lane + 64 * k < C::NF) out[t.out + lane + 64 * k] = acc[k];
+    } else if (mode == 1) {
+        double m = 0.0;
+        if constexpr (LA == LC && LB == LD) {
+#pragma unroll
+            for (int k = 0; k < S::NPL; k++) {
+                const int f = lane + 64 * k;
+                if (f >= C::NF) continue;
+                const int fa = f % C::NFA, t1 = f / C::NFA;
+                const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
+                const int fc = t2 % C::NFC, fd = t2 / C::NFC;
+                if (fa == fc && fb == fd) m = fmax(m, fabs(acc[k]));
+            }
+        }
+        red[lane] = m;
+        __syncthreads();
+        if (lane == 0) {
+            double mm = 0.0;
+            for (int e = 0; e < 64; e++) mm = fmax(mm, red[e]);
+            out[t.out] = sqrt(mm);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < S::NPL; k++) {
+            const int f = lane + 64 * k;
+            if (f >= C::NF) continue;
+            const int fa = f % C::NFA, t1 = f / C::NFA;
+            const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
+            const int fc = t2 % C::NFC, fd = t2 / C::NFC;
+            const double v = acc[k];
+            const size_t a = bra.aoa + fa, b = bra.aob + fb, c = ket.aoa + fc, d = ket.aob + fd;
+            out[((a * n + b) * n + c) * n + d] = v;
+            out[((b * n + a) * n + c) * n + d] = v;
+            out[((a * n + b) * n + d) * n + c] = v;
+            out[((b * n + a) * n + d) * n + c] = v;
+            out[((c * n + d) * n + a) * n + b] = v;
+            out[((d * n + c) * n + a) * n + b] = v;
+            out[((c * n + d) * n + b) * n + a] = v;
+            out[((d * n + c) * n + b) * n + a] = v;
+        }
+    }
+}
+
 // ---------------------------------------------------------- list kernel --
 // mode 0: write the block to out[task.out .. + NF) (a fastest)
 // mode 1: write sqrt(max_ab |(ab|ab)|) to out[task.out] (Schwarz diagonal)
@@ -245,21 +506,26 @@ __global__ void symmetrize_kernel(const double *__restrict__ M, double *__restri
 }
 
 // ---------------------------------------------------------- dispatch --
-// Pair classes (la <= lb): 0:(0,0) 1:(0,1) 2:(1,1) 3:(0,2) 4:(1,2) 5:(2,2)
-#define LR_FOR_CLASS_PAIRS(X)                                                                                          \
-    X(0, 0, 0, 0) X(0, 1, 0, 0) X(0, 1, 0, 1) X(1, 1, 0, 0) X(1, 1, 0, 1) X(1, 1, 1, 1)
-
-static int cls_index(int la, int lb)
-{
-    static const int tab[3][3] = {{0, 1, 3}, {-1, 2, 4}, {-1, -1, 5}};
-    return tab[la][lb];
-}
+// Pair classes (la <= lb): 0:(0,0) 1:(0,1) 2:(1,1) 3:(0,2) 4:(1,2) 5:(2,2);
+// bra class >= ket class.  REG: contracted block in registers (NF <= 108);
+// TEAM: one wavefront per quartet.
+#define LR_REG_CLASSES(X)                                                                                              \
+    X(0, 0, 0, 0) X(0, 1, 0, 0) X(0, 1, 0, 1) X(1, 1, 0, 0) X(1, 1, 0, 1) X(1, 1, 1, 1) X(0, 2, 0, 0) X(0, 2, 0, 1)   \
+        X(0, 2, 1, 1) X(0, 2, 0, 2) X(1, 2, 0, 0) X(1, 2, 0, 1) X(1, 2, 0, 2) X(2, 2, 0, 0) X(2, 2, 0, 1)
+#define LR_TEAM_CLASSES(X) X(1, 2, 1, 1) X(1, 2, 1, 2) X(2, 2, 1, 1) X(2, 2, 0, 2) X(2, 2, 1, 2) X(2, 2, 2, 2)
+// the list kernel keeps fewer live values than the J/K kernel but no fewer
+// outputs; these two also go to the team form there
+#define LR_LIST_REG_CLASSES(X)                                                                                         \
+    X(0, 0, 0, 0) X(0, 1, 0, 0) X(0, 1, 0, 1) X(1, 1, 0, 0) X(1, 1, 0, 1) X(1, 1, 1, 1) X(0, 2, 0, 0) X(0, 2, 0, 1)   \
+        X(0, 2, 1, 1) X(0, 2, 0, 2) X(1, 2, 0, 0) X(1, 2, 0, 1) X(2, 2, 0, 0)
+#define LR_LIST_TEAM_CLASSES(X) LR_TEAM_CLASSES(X) X(1, 2, 0, 2) X(2, 2, 0, 1)
 
 bool class_supported(int la, int lb, int lc, int ld)
 {
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d) return true;
-    LR_FOR_CLASS_PAIRS(X)
+    LR_REG_CLASSES(X)
+    LR_TEAM_CLASSES(X)
 #undef X
     return false;
 }
@@ -273,7 +539,14 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int nt
         hipLaunchKernelGGL((jk_kernel<a, b, c, d>), dim3(ntask), dim3(64), 0, st, tasks, B, D, M, nao);               \
         return hipGetLastError();                                                                                      \
     }
-    LR_FOR_CLASS_PAIRS(X)
+    LR_REG_CLASSES(X)
+#undef X
+#define X(a, b, c, d)                                                                                                  \
+    if (la == a && lb == b && lc == c && ld == d) {                                                                    \
+        hipLaunchKernelGGL((jk_team_kernel<a, b, c, d>), dim3(ntask), dim3(64), 0, st, tasks, B, D, M, nao);          \
+        return hipGetLastError();                                                                                      \
+    }
+    LR_TEAM_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
 }
@@ -289,7 +562,15 @@ hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, in
                            nao);                                                                                       \
         return hipGetLastError();                                                                                      \
     }
-    LR_FOR_CLASS_PAIRS(X)
+    LR_LIST_REG_CLASSES(X)
+#undef X
+#define X(a, b, c, d)                                                                                                  \
+    if (la == a && lb == b && lc == c && ld == d) {                                                                    \
+        hipLaunchKernelGGL((eri_list_team_kernel<a, b, c, d>), dim3(ntask), dim3(64), 0, st, tasks, ntask, B, out,    \
+                           mode, nao);                                                                                 \
+        return hipGetLastError();                                                                                      \
+    }
+    LR_LIST_TEAM_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
 }
