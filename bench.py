@@ -107,6 +107,10 @@ def main():
         ctx.fock(P)
         ev_ms.append(ctx.stats()["kernel_ms"])
     st = ctx.stats()
+    classes = [{"class": "(%d%d|%d%d)" % (int(c["la"]), int(c["lb"]), int(c["lc"]), int(c["ld"])),
+                "quartets": int(c["quartets"]), "prim_quartets": int(c["prim_quartets"]),
+                "lane_util": c["quartets"] / max(c["lanes"], 1), "ms": c["ms"],
+                "tflops": c["flops"] / max(c["ms"], 1e-9) / 1e9} for c in ctx.class_stats()]
     ms_per_step = wall / args.steps * 1e3
     quartets = st["quartets"]
     jk_ms = float(np.mean(ev_ms))
@@ -133,6 +137,7 @@ def main():
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": "jk_kernel (all angular classes of one build)",
                      "kernel_ms": jk_ms, "flops_per_launch": st["flops"]},
+        "classes": classes,
     }
     if not args.no_cpu:
         rate, t_full, nq, used = cpu_baseline(m, P, args.cpu_seconds)

@@ -105,6 +105,13 @@ int librint_schwarz(librint_ctx *ctx, double *out);
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 
+/* Per angular-class launch of the current plan, 10 values each:
+ * la, lb, lc, ld, quartets, primitive quartets, algorithmic flops, lanes
+ * launched (64 x tasks), device ms of the launch in the last host-side build
+ * (librint_fock), task count.  Returns the number of launches (out may be
+ * NULL to query it); writes at most n launches. */
+int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);
+
 /* RHF on the context.  guess: 0 identity (reference), 1 core Hamiltonian,
  * 2 caller density in P_inout.  diis: 0/1.  On return P_inout (nao*nao,
  * may be NULL) holds the final density and *iters the cycle count.

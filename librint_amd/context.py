@@ -80,6 +80,14 @@ class FockContext:
         library.librint_ctx_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 8)
         return dict(zip(STAT_KEYS, out.tolist()))
 
+    def class_stats(self):
+        """Per angular-class launch: counts, flops and the last build's device ms."""
+        n = library.librint_ctx_class_stats(self.h, None, 0)
+        out = np.zeros(10 * max(n, 1))
+        library.librint_ctx_class_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+        keys = ("la", "lb", "lc", "ld", "quartets", "prim_quartets", "flops", "lanes", "ms", "tasks")
+        return [dict(zip(keys, out[10 * k:10 * k + 10].tolist())) for k in range(n)]
+
     def scf(self, imax=200, conv=1e-6, guess=0, diis=False, P0=None):
         P = np.zeros((self.nao, self.nao)) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
         it = ctypes.c_int(0)

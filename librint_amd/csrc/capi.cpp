@@ -304,6 +304,21 @@ int librint_ctx_stats(const librint_ctx *ctx, double *out, int n)
     return k;
 }
 
+int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n)
+{
+    if (!ctx) return -1;
+    const auto &L = ctx->ctx->launches();
+    if (!out) return (int)L.size();
+    int k = 0;
+    for (; k < (int)L.size() && k < n; k++) {
+        const ClassLaunch &c = L[k];
+        const double v[10] = {(double)c.la, (double)c.lb, (double)c.lc, (double)c.ld, (double)c.quartets,
+                              (double)c.prim_quartets, c.flops, (double)c.lanes, (double)c.ms, (double)c.count};
+        for (int j = 0; j < 10; j++) out[10 * k + j] = v[j];
+    }
+    return (int)L.size();
+}
+
 double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess, int diis, double *P_inout,
                    int *iters)
 {

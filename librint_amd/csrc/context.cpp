@@ -107,6 +107,7 @@ Context::~Context()
     dev_free(d_G_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
 }
 
 DevBasis Context::devb() const
@@ -161,11 +162,13 @@ void Context::build_pairs()
                     if (cce > expcutoff) continue;
                     PrimPair r;
                     r.e = e;
-                    r.K = cp * cq * cf * std::exp(-eij);
+                    // sqrt(2 pi^{5/2}): the quartet prefactor splits evenly over the two pairs
+                    r.K = 5.914967172795612 * cp * cq * cf * std::exp(-eij) / e;
+                    r.h = 0.5 / e;
                     r.pbx = (ap * A.r[0] + aq * Bs.r[0]) / e - Bs.r[0];
                     r.pby = (ap * A.r[1] + aq * Bs.r[1]) / e - Bs.r[1];
                     r.pbz = (ap * A.r[2] + aq * Bs.r[2]) / e - Bs.r[2];
-                    r.pad0 = r.pad1 = r.pad2 = 0.0;
+                    r.pad0 = r.pad1 = 0.0;
                     pp_.push_back(r);
                 }
             P.npp = (int)pp_.size() - P.off;
@@ -294,6 +297,7 @@ void Context::build_tasks()
     stats_.candidates = np * (np + 1) / 2;
     // group bin pairs by class pair
     std::map<std::pair<int, int>, std::vector<std::tuple<long long, JKTask>>> by_cls;
+    std::map<std::pair<int, int>, ClassLaunch> cstat;
     for (int X = 0; X < nb; X++)
         for (int Y = 0; Y <= X; Y++) {
             const Bin &bx = bins_[X], &by = bins_[Y];
@@ -322,6 +326,11 @@ void Context::build_tasks()
                 stats_.prim_quartets += (long long)n * bx.npp * by.npp;
                 stats_.integrals += (long long)n * nf;
                 stats_.flops += (double)n * bx.npp * by.npp * fp + 12.0 * nf * n;
+                ClassLaunch &cs = cstat[{bx.cls, by.cls}];
+                cs.quartets += n;
+                cs.prim_quartets += (long long)n * bx.npp * by.npp;
+                cs.flops += (double)n * bx.npp * by.npp * fp + 12.0 * nf * n;
+                cs.lanes += 64LL * ((n + 63) / 64);
                 for (int k0 = 0; k0 < n; k0 += 64) {
                     JKTask t{p, by.first + k0, std::min(64, n - k0), 0};
                     lst.emplace_back((long long)bx.npp * by.npp, t);
@@ -335,7 +344,7 @@ void Context::build_tasks()
                          [](const std::tuple<long long, JKTask> &a, const std::tuple<long long, JKTask> &b) {
                              return std::get<0>(a) > std::get<0>(b);
                          });
-        ClassLaunch L;
+        ClassLaunch L = cstat[kv.first];
         cls_ls(kv.first.first, L.la, L.lb);
         cls_ls(kv.first.second, L.lc, L.ld);
         L.first = (int)tasks.size();
@@ -344,6 +353,9 @@ void Context::build_tasks()
         if (L.count) launches_.push_back(L);
     }
     stats_.launches = (int)launches_.size();
+    for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
+    evc_.assign(launches_.size(), nullptr);
+    for (auto &e : evc_) check(hipEventCreate(&e), "hipEventCreate");
     dev_free(d_tasks_);
     d_tasks_ = dev_upload(tasks);
 }
@@ -355,8 +367,11 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
     const DevBasis B = devb();
     check(hipEventRecord(ev0_, st), "hipEventRecord");
-    for (const ClassLaunch &L : launches_)
+    for (size_t k = 0; k < launches_.size(); k++) {
+        const ClassLaunch &L = launches_[k];
         check(launch_jk(L.la, L.lb, L.lc, L.ld, d_tasks_ + L.first, L.count, B, D_dev, d_M_, nao_, st), "jk launch");
+        check(hipEventRecord(evc_[k], st), "hipEventRecord");
+    }
     check(hipEventRecord(ev1_, st), "hipEventRecord");
     check(launch_symmetrize(d_M_, G_dev, nao_, st), "symmetrize launch");
 }
@@ -370,6 +385,17 @@ void Context::fock_host(const double *D, double *G)
     fock_device(d_D_, d_G_, 0);
     check(hipMemcpy(G, d_G_, sizeof(double) * nn, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     check(hipEventElapsedTime(&stats_.last_kernel_ms, ev0_, ev1_), "hipEventElapsedTime");
+    collect_class_times();
+}
+
+void Context::collect_class_times()
+{
+    hipEvent_t prev = ev0_;
+    for (size_t k = 0; k < launches_.size(); k++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, prev, evc_[k]) == hipSuccess) launches_[k].ms = ms;
+        prev = evc_[k];
+    }
 }
 
 void Context::tensor_host(double *R)

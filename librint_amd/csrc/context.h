@@ -34,6 +34,9 @@ struct Stats {
 struct ClassLaunch {
     int la, lb, lc, ld;
     int first, count;  // range in the device task array
+    long long quartets = 0, prim_quartets = 0, lanes = 0;
+    double flops = 0;
+    float ms = 0.f;    // device time of this launch in the last build
 };
 
 class Context {
@@ -68,6 +71,8 @@ public:
     const std::vector<double> &env() const { return env_; }
     const std::vector<int> &bas() const { return bas_; }
     const std::vector<ClassLaunch> &launches() const { return launches_; }
+    // per-launch device times of the last build (host wrapper synchronises)
+    void collect_class_times();
 
 private:
     void build_pairs();
@@ -107,6 +112,7 @@ private:
     JKTask *d_tasks_ = nullptr;
     double *d_M_ = nullptr, *d_D_ = nullptr, *d_G_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    std::vector<hipEvent_t> evc_;  // one event after each class launch
     DevBasis devb() const;
 };
 

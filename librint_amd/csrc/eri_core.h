@@ -138,14 +138,38 @@ __device__ __forceinline__ void gout_acc(A *eri, const T *gx, const T *gy, const
 }
 
 // Primitive-pair record (one per surviving primitive pair of a shell pair).
-//   e = ai + aj, K = ca*cb*fac_sp(la)*fac_sp(lb)*exp(-ai aj/e |Ra-Rb|^2),
+//   e  = ai + aj
+//   K  = sqrt(2 pi^{5/2}) ca cb fac_sp(la) fac_sp(lb) exp(-ai aj/e |Ra-Rb|^2) / e
+//        so that the quartet prefactor 2 pi^{5/2} K_ij K_kl / (e_ij e_kl
+//        sqrt(e_ij + e_kl)) of reference src/g2e.rs:5547-5560 is K_p K_q rsqrt(e_p + e_q)
+//   h  = 1 / (2 e)
 //   pb = P - Rb  (P the Gaussian product centre, Rb the VRR centre).
 struct PrimPair {
     double e;
     double K;
+    double h;
     double pbx, pby, pbz;
-    double pad0, pad1, pad2;
+    double pad0, pad1;
 };
+
+// Reciprocal and reciprocal square root for positive normal arguments:
+// hardware estimate plus two Newton steps (relative error ~1 ulp).
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double r = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, r, y);
+    r = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, r, y);
+}
+__device__ __forceinline__ double rcp_pos(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+}
 
 // One primitive quartet, all roots, accumulated into eri (double precision).
 // rbd = Rb - Rd, rab = Rb - Ra, rcd = Rd - Rc.
@@ -154,14 +178,13 @@ __device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p
                                                   const double rab[3], const double rcd[3])
 {
     using C = Class<LA, LB, LC, LD>;
-    constexpr double TWO_PI_52 = 34.98683665524972497; // 2 pi^{5/2}
     const double aij = p.e, akl = q.e;
-    const double a1 = aij * akl;
-    const double rs = 1.0 / (aij + akl);
-    const double a0 = a1 * rs;
+    const double rsq = rsqrt_pos(aij + akl);
+    const double rs = rsq * rsq;
+    const double a0 = aij * akl * rs;
     const double pq[3] = {p.pbx - q.pbx + rbd[0], p.pby - q.pby + rbd[1], p.pbz - q.pbz + rbd[2]};
     const double rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
-    const double fac1 = TWO_PI_52 * p.K * q.K * sqrt(rs) / a1;
+    const double fac1 = p.K * q.K * rsq;
     double v[C::NROOTS], w[C::NROOTS];
     rys_nodes<C::NROOTS>(a0 * rr, v, w);
 #pragma unroll
@@ -170,8 +193,8 @@ __device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p
         const double t2 = v[r];
         const double tr = t2 * rs;
         const double b00 = 0.5 * tr;
-        const double b10 = 0.5 / aij * (1.0 - akl * tr);
-        const double b01 = 0.5 / akl * (1.0 - aij * tr);
+        const double b10 = p.h * (1.0 - akl * tr);
+        const double b01 = q.h * (1.0 - aij * tr);
         const double ta = akl * tr; // weight of (P-Q) in the bra shift
         const double tc = aij * tr;
         double gx[C::GS], gy[C::GS], gz[C::GS];

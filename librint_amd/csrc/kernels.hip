@@ -216,14 +216,13 @@ __device__ __forceinline__ void team_chain(double *g, int c, const PrimPair &p, 
                                            const double rab[3], const double rcd[3])
 {
     using C = Class<LA, LB, LC, LD>;
-    constexpr double TWO_PI_52 = 34.98683665524972497;
     const double aij = p.e, akl = q.e;
-    const double a1 = aij * akl;
-    const double rs = 1.0 / (aij + akl);
-    const double a0 = a1 * rs;
+    const double rsq = rsqrt_pos(aij + akl);
+    const double rs = rsq * rsq;
+    const double a0 = aij * akl * rs;
     const double pqx = p.pbx - q.pbx + rbd[0], pqy = p.pby - q.pby + rbd[1], pqz = p.pbz - q.pbz + rbd[2];
     const double rr = pqx * pqx + pqy * pqy + pqz * pqz;
-    const double fac1 = TWO_PI_52 * p.K * q.K * sqrt(rs) / a1;
+    const double fac1 = p.K * q.K * rsq;
     double v[C::NROOTS], w[C::NROOTS];
     rys_nodes<C::NROOTS>(a0 * rr, v, w);
     const int r = c / 3, ax = c - 3 * (c / 3);
@@ -236,8 +235,8 @@ __device__ __forceinline__ void team_chain(double *g, int c, const PrimPair &p, 
         }
     const double tr = t2 * rs;
     const double b00 = 0.5 * tr;
-    const double b10 = 0.5 / aij * (1.0 - akl * tr);
-    const double b01 = 0.5 / akl * (1.0 - aij * tr);
+    const double b10 = p.h * (1.0 - akl * tr);
+    const double b01 = q.h * (1.0 - aij * tr);
     const double pq = sel3(ax, pqx, pqy, pqz);
     const double cn = sel3(ax, p.pbx, p.pby, p.pbz) - akl * tr * pq;
     const double cm = sel3(ax, q.pbx, q.pby, q.pbz) + aij * tr * pq;

@@ -36,8 +36,17 @@ RYS_HD inline void rys_nodes(double x, double *v, double *w)
     static_assert(N >= 1 && N <= RYS_NMAX, "root count");
     constexpr int NV = 2 * N;
     if (x >= (double)RYS_XCUT[N]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        double rs = __builtin_amdgcn_rsq(x);
+        double e = fma(-x * rs, rs, 1.0);
+        rs = fma(0.5 * rs, e, rs);
+        e = fma(-x * rs, rs, 1.0);
+        rs = fma(0.5 * rs, e, rs);
+        const double rx = rs * rs;
+#else
         const double rx = 1.0 / x;
         const double rs = sqrt(rx);
+#endif
 #pragma unroll
         for (int r = 0; r < N; r++) {
             v[r] = RYS_HERM_NODE2[RYS_HOFF[N] + r] * rx;
