@@ -102,6 +102,7 @@ Context::~Context()
     dev_free(d_pp_);
     dev_free(d_ao_off_);
     dev_free(d_tasks_);
+    dev_free(d_counters_);
     dev_free(d_M_);
     dev_free(d_D_);
     dev_free(d_G_);
@@ -358,6 +359,8 @@ void Context::build_tasks()
     for (auto &e : evc_) check(hipEventCreate(&e), "hipEventCreate");
     dev_free(d_tasks_);
     d_tasks_ = dev_upload(tasks);
+    dev_free(d_counters_);
+    check(hipMalloc(&d_counters_, sizeof(int) * std::max<size_t>(launches_.size(), 1)), "hipMalloc");
 }
 
 void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
@@ -366,10 +369,12 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
     check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
     const DevBasis B = devb();
+    check(launch_zero_counters(d_counters_, (int)launches_.size(), st), "zero counters");
     check(hipEventRecord(ev0_, st), "hipEventRecord");
     for (size_t k = 0; k < launches_.size(); k++) {
         const ClassLaunch &L = launches_[k];
-        check(launch_jk(L.la, L.lb, L.lc, L.ld, d_tasks_ + L.first, L.count, B, D_dev, d_M_, nao_, st), "jk launch");
+        check(launch_jk(L.la, L.lb, L.lc, L.ld, d_tasks_ + L.first, L.count, d_counters_ + k, B, D_dev, d_M_, nao_, st),
+              "jk launch");
         check(hipEventRecord(evc_[k], st), "hipEventRecord");
     }
     check(hipEventRecord(ev1_, st), "hipEventRecord");

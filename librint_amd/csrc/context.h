@@ -110,6 +110,7 @@ private:
     PrimPair *d_pp_ = nullptr;
     int *d_ao_off_ = nullptr;
     JKTask *d_tasks_ = nullptr;
+    int *d_counters_ = nullptr;  // one work counter per class launch
     double *d_M_ = nullptr, *d_D_ = nullptr, *d_G_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<hipEvent_t> evc_;  // one event after each class launch

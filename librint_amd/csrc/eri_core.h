@@ -171,11 +171,68 @@ __device__ __forceinline__ double rcp_pos(double x)
     return fma(y, e, y);
 }
 
+// Rys nodes/weights from a table block of one root count staged in LDS:
+// rt = [seg][c][2N] (v then w), or for WONLY = [seg][c] weights only.
+template <int N, bool WONLY>
+__device__ __forceinline__ void rys_nodes_lds(double x, const double *rt, double *v, double *w)
+{
+    constexpr int NV = WONLY ? 1 : 2 * N;
+    if (x >= (double)RYS_NSEG[N]) {
+        const double rs = rsqrt_pos(x);
+        const double rx = rs * rs;
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+            v[r] = WONLY ? 0.0 : RYS_HERM_NODE2[RYS_HOFF[N] + r] * rx;
+            w[r] = RYS_HERM_W[RYS_HOFF[N] + r] * rs;
+        }
+        return;
+    }
+    const int k = (int)x;
+    const double s = 2.0 * (x - (double)k) - 1.0;
+    const double s2 = 2.0 * s;
+    const double *c = rt + k * (RYS_DEG + 1) * NV;
+    double b1[NV], b2[NV];
+#pragma unroll
+    for (int j = 0; j < NV; j++) {
+        b1[j] = 0.0;
+        b2[j] = 0.0;
+    }
+#pragma unroll
+    for (int m = RYS_DEG; m >= 1; m--) {
+#pragma unroll
+        for (int j = 0; j < NV; j++) {
+            const double b0 = fma(s2, b1[j], c[m * NV + j] - b2[j]);
+            b2[j] = b1[j];
+            b1[j] = b0;
+        }
+    }
+    if constexpr (WONLY) {
+        w[0] = fma(s, b1[0], c[0] - b2[0]);
+        v[0] = 0.0;
+    } else {
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            v[j] = fma(s, b1[j], c[j] - b2[j]);
+            w[j] = fma(s, b1[N + j], c[N + j] - b2[N + j]);
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+struct RysLds {
+    using C = Class<LA, LB, LC, LD>;
+    static constexpr int N = C::NROOTS;
+    static constexpr bool WONLY = (C::NMAX + C::MMAX) == 0;
+    static constexpr int NV = WONLY ? 1 : 2 * N;
+    static constexpr int LEN = RYS_NSEG[N] * (RYS_DEG + 1) * NV;
+};
+
 // One primitive quartet, all roots, accumulated into eri (double precision).
 // rbd = Rb - Rd, rab = Rb - Ra, rcd = Rd - Rc.
-template <int LA, int LB, int LC, int LD>
+// LDS_TAB: Rys table block staged in LDS at rt (see RysLds), else global table.
+template <int LA, int LB, int LC, int LD, bool LDS_TAB>
 __device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p, const PrimPair &q, const double rbd[3],
-                                                  const double rab[3], const double rcd[3])
+                                                  const double rab[3], const double rcd[3], const double *rt)
 {
     using C = Class<LA, LB, LC, LD>;
     const double aij = p.e, akl = q.e;
@@ -186,7 +243,10 @@ __device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p
     const double rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
     const double fac1 = p.K * q.K * rsq;
     double v[C::NROOTS], w[C::NROOTS];
-    rys_nodes<C::NROOTS>(a0 * rr, v, w);
+    if constexpr (LDS_TAB)
+        rys_nodes_lds<C::NROOTS, RysLds<LA, LB, LC, LD>::WONLY>(a0 * rr, rt, v, w);
+    else
+        rys_nodes<C::NROOTS>(a0 * rr, v, w);
 #pragma unroll
     for (int r = 0; r < C::NROOTS; r++) {
         // v = t^2;  b00 = t^2/(2(aij+akl)), b10 = (1 - akl t^2/(aij+akl))/(2 aij), ...

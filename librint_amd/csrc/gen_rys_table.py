@@ -173,6 +173,8 @@ def main():
         f.write("// `__device__ static const` in device code).\n")
         f.write("#define RYS_NMAX %d\n#define RYS_DEG %d\n#define RYS_SEGLEN %.17g\n" % (NMAX, DEG, SEGLEN))
         f.write("RYS_STORAGE int RYS_XCUT[%d] = {0, %s};\n" % (NMAX + 1, ", ".join(str(xcut[n]) for n in range(1, NMAX + 1))))
+        f.write("#ifndef RYS_NSEG_DEFINED\n#define RYS_NSEG_DEFINED\n")
+        f.write("static constexpr int RYS_NSEG[%d] = {0, %s};\n#endif\n" % (NMAX + 1, ", ".join(str(nseg[n]) for n in range(1, NMAX + 1))))
         f.write("RYS_STORAGE int RYS_OFF[%d] = {0, %s};\n" % (NMAX + 1, ", ".join(str(off[n]) for n in range(1, NMAX + 1))))
         hoff = [0]
         for n in range(1, NMAX + 1):

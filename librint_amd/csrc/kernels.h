@@ -38,8 +38,10 @@ struct ListTask {
 };
 
 bool class_supported(int la, int lb, int lc, int ld);
-hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, const DevBasis &B,
+// counter: one int per launch, zero before the launch (launch_zero_counters)
+hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, int *counter, const DevBasis &B,
                      const double *D, double *M, int nao, hipStream_t st);
+hipError_t launch_zero_counters(int *p, int n, hipStream_t st);
 hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
                        double *out, int mode, int nao, hipStream_t st);
 hipError_t launch_symmetrize(const double *M, double *G, int nao, hipStream_t st);

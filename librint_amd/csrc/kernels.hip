@@ -14,6 +14,8 @@
 // screen are additions of this implementation.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "eri_core.h"
 #include "kernels.h"
 
@@ -65,7 +67,7 @@ __device__ __forceinline__ void contracted_fp64(double *eri, const DevBasis &B, 
         const PrimPair p = B.pp[bra.off + ib];
         for (int ik = 0; ik < ket.npp; ik++) {
             const PrimPair q = B.pp[ket.off + ik];
-            prim_quartet_fp64<LA, LB, LC, LD>(eri, p, q, rbd, bra.rab, ket.rab);
+            prim_quartet_fp64<LA, LB, LC, LD, false>(eri, p, q, rbd, bra.rab, ket.rab, nullptr);
         }
     }
 }
@@ -77,20 +79,33 @@ __device__ __forceinline__ void contracted_fp64(double *eri, const DevBasis &B, 
 //     M[a][c] -= s/2 D[b][d] v      M[b][c] -= s/2 D[a][d] v
 //     M[a][d] -= s/2 D[b][c] v      M[b][d] -= s/2 D[a][c] v
 // which sums the J and K contributions of all 8 index images.
+//
+// Persistent form: 256-thread workgroups stage the Rys table block of the
+// class's root count in LDS once, then each wavefront pulls tasks from a
+// per-launch counter.  In a task the bra pair is wavefront-uniform (its
+// primitive records become scalar loads) and each lane keeps its ket
+// primitive record in registers while the bra primitives stream past.
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(64) void jk_kernel(const JKTask *__restrict__ tasks, DevBasis B,
-                                                const double *__restrict__ D, double *__restrict__ M, int nao)
+__device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, const double *__restrict__ D,
+                                        double *__restrict__ M, int nao, int lane, const double *rt)
 {
     using C = Class<LA, LB, LC, LD>;
-    const JKTask t = tasks[blockIdx.x];
-    const int lane = threadIdx.x;
     const bool active = lane < t.nket;
     const int ketp = t.ket0 + (active ? lane : 0);
     const PairView bra = load_pair(B, t.bra);
     const PairView ket = load_pair(B, ketp);
 
     double eri[C::NF];
-    contracted_fp64<LA, LB, LC, LD>(eri, B, bra, ket);
+#pragma unroll
+    for (int f = 0; f < C::NF; f++) eri[f] = 0.0;
+    const double rbd[3] = {bra.rb[0] - ket.rb[0], bra.rb[1] - ket.rb[1], bra.rb[2] - ket.rb[2]};
+    for (int ik = 0; ik < ket.npp; ik++) {
+        const PrimPair q = B.pp[ket.off + ik];
+        for (int ib = 0; ib < bra.npp; ib++) {
+            const PrimPair p = B.pp[bra.off + ib];
+            prim_quartet_fp64<LA, LB, LC, LD, true>(eri, p, q, rbd, bra.rab, ket.rab, rt);
+        }
+    }
 
     double s = active ? 1.0 : 0.0;
     if (bra.sa == bra.sb) s *= 0.5;
@@ -180,6 +195,27 @@ __global__ __launch_bounds__(64) void jk_kernel(const JKTask *__restrict__ tasks
                            eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
             atomicAdd(&M[(bra.aob + fb) * n + ket.aob + fd], h * acc);
         }
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ tasks, int ntask, int *__restrict__ counter,
+                                                 DevBasis B, const double *__restrict__ D, double *__restrict__ M, int nao)
+{
+    using R = RysLds<LA, LB, LC, LD>;
+    __shared__ double rt[R::LEN];
+    {
+        const double *src = RYS_CHEB + RYS_OFF[R::N];
+        for (int i = threadIdx.x; i < R::LEN; i += blockDim.x) rt[i] = R::WONLY ? src[2 * i + 1] : src[i];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(counter, 1);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= ntask) break;
+        jk_task<LA, LB, LC, LD>(tasks[t], B, D, M, nao, lane, rt);
     }
 }
 
@@ -529,15 +565,42 @@ bool class_supported(int la, int lb, int lc, int ld)
     return false;
 }
 
-hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, const DevBasis &B,
+template <int LA, int LB, int LC, int LD>
+static hipError_t launch_jk_persistent(const JKTask *tasks, int ntask, int *counter, const DevBasis &B, const double *D,
+                                       double *M, int nao, hipStream_t st)
+{
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, ncu = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_kernel<LA, LB, LC, LD>, 256, 0);
+        grid = std::max(1, ncu) * std::max(1, per);
+    }
+    const int g = std::min(grid, (ntask + 3) / 4);
+    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD>), dim3(g), dim3(256), 0, st, tasks, ntask, counter, B, D, M, nao);
+    return hipGetLastError();
+}
+
+__global__ void zero_ints_kernel(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+hipError_t launch_zero_counters(int *p, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, int *counter, const DevBasis &B,
                      const double *D, double *M, int nao, hipStream_t st)
 {
     if (ntask == 0) return hipSuccess;
 #define X(a, b, c, d)                                                                                                  \
-    if (la == a && lb == b && lc == c && ld == d) {                                                                    \
-        hipLaunchKernelGGL((jk_kernel<a, b, c, d>), dim3(ntask), dim3(64), 0, st, tasks, B, D, M, nao);               \
-        return hipGetLastError();                                                                                      \
-    }
+    if (la == a && lb == b && lc == c && ld == d) return launch_jk_persistent<a, b, c, d>(tasks, ntask, counter, B, D, M, nao, st);
     LR_REG_CLASSES(X)
 #undef X
 #define X(a, b, c, d)                                                                                                  \
