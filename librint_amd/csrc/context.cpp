@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -86,6 +87,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
         }
     }
     nao_ = ao;
+    no_group_ = getenv("LIBRINT_NO_GROUP") != nullptr;
     check(hipEventCreate(&ev0_), "hipEventCreate");
     check(hipEventCreate(&ev1_), "hipEventCreate");
     build_pairs();
@@ -102,6 +104,7 @@ Context::~Context()
     dev_free(d_pp_);
     dev_free(d_ao_off_);
     dev_free(d_tasks_);
+    dev_free(d_groups_);
     dev_free(d_counters_);
     dev_free(d_M_);
     dev_free(d_D_);
@@ -291,28 +294,33 @@ static double flops_prim(int la, int lb, int lc, int ld)
 void Context::build_tasks()
 {
     const int nb = (int)bins_.size();
-    std::vector<JKTask> tasks;
     launches_.clear();
     stats_ = Stats();
     const long long np = (long long)pairs_.size();
     stats_.candidates = np * (np + 1) / 2;
-    // group bin pairs by class pair
-    std::map<std::pair<int, int>, std::vector<std::tuple<long long, JKTask>>> by_cls;
-    std::map<std::pair<int, int>, ClassLaunch> cstat;
-    for (int X = 0; X < nb; X++)
-        for (int Y = 0; Y <= X; Y++) {
-            const Bin &bx = bins_[X], &by = bins_[Y];
-            int la, lb, lc, ld;
-            cls_ls(bx.cls, la, lb);
-            cls_ls(by.cls, lc, ld);
-            if (!class_supported(la, lb, lc, ld))
-                throw std::runtime_error("angular class not supported by the GPU kernels");
-            const double fp = flops_prim(la, lb, lc, ld);
-            const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-            auto &lst = by_cls[{bx.cls, by.cls}];
-            for (int i = 0; i < bx.count; i++) {
-                const int p = bx.first + i;
-                const double Qp = pairs_[p].Q;
+    // A group = all tasks of one bra pair for one ket class (capped so that a
+    // heavy bra is spread over a few workgroups).
+    const int GROUP_MAX_TASKS = 32;
+    struct Acc {
+        std::vector<JKTask> tasks;
+        std::vector<std::pair<double, JKGroup>> groups;
+        ClassLaunch stat;
+    };
+    std::map<std::pair<int, int>, Acc> acc;
+    for (int X = 0; X < nb; X++) {
+        const Bin &bx = bins_[X];
+        int la, lb;
+        cls_ls(bx.cls, la, lb);
+        for (int i = 0; i < bx.count; i++) {
+            const int p = bx.first + i;
+            const double Qp = pairs_[p].Q;
+            std::map<int, int> open;  // ket class -> index of the open group
+            for (int Y = 0; Y <= X; Y++) {
+                const Bin &by = bins_[Y];
+                int lc, ld;
+                cls_ls(by.cls, lc, ld);
+                if (!class_supported(la, lb, lc, ld))
+                    throw std::runtime_error("angular class not supported by the GPU kernels");
                 // kets of bin Y with Q_p Q_k > tau: a prefix (Q descending)
                 int lo = 0, hi = by.count;
                 while (lo < hi) {
@@ -323,34 +331,56 @@ void Context::build_tasks()
                 int n = lo;
                 if (X == Y) n = std::min(n, i + 1);
                 if (n <= 0) continue;
+                const double fp = flops_prim(la, lb, lc, ld);
+                const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+                const double fl = (double)n * bx.npp * by.npp * fp + 12.0 * nf * n;
                 stats_.quartets += n;
                 stats_.prim_quartets += (long long)n * bx.npp * by.npp;
                 stats_.integrals += (long long)n * nf;
-                stats_.flops += (double)n * bx.npp * by.npp * fp + 12.0 * nf * n;
-                ClassLaunch &cs = cstat[{bx.cls, by.cls}];
-                cs.quartets += n;
-                cs.prim_quartets += (long long)n * bx.npp * by.npp;
-                cs.flops += (double)n * bx.npp * by.npp * fp + 12.0 * nf * n;
-                cs.lanes += 64LL * ((n + 63) / 64);
+                stats_.flops += fl;
+                Acc &A = acc[{bx.cls, by.cls}];
+                A.stat.quartets += n;
+                A.stat.prim_quartets += (long long)n * bx.npp * by.npp;
+                A.stat.flops += fl;
+                A.stat.lanes += 64LL * ((n + 63) / 64);
                 for (int k0 = 0; k0 < n; k0 += 64) {
-                    JKTask t{p, by.first + k0, std::min(64, n - k0), 0};
-                    lst.emplace_back((long long)bx.npp * by.npp, t);
+                    auto it = open.find(by.cls);
+                    if (it == open.end() || A.groups[it->second].second.count >= GROUP_MAX_TASKS) {
+                        A.groups.push_back({0.0, JKGroup{p, (int)A.tasks.size(), 0, 0}});
+                        open[by.cls] = (int)A.groups.size() - 1;
+                        it = open.find(by.cls);
+                    }
+                    const int nk = std::min(64, n - k0);
+                    A.tasks.push_back(JKTask{p, by.first + k0, nk, 0});
+                    A.groups[it->second].second.count++;
+                    A.groups[it->second].first += (double)nk * bx.npp * by.npp;
                 }
             }
         }
-    for (auto &kv : by_cls) {
-        auto &lst = kv.second;
-        // heavy tasks first for load balance
-        std::stable_sort(lst.begin(), lst.end(),
-                         [](const std::tuple<long long, JKTask> &a, const std::tuple<long long, JKTask> &b) {
-                             return std::get<0>(a) > std::get<0>(b);
+    }
+    std::vector<JKTask> tasks;
+    std::vector<JKGroup> groups;
+    for (auto &kv : acc) {
+        Acc &A = kv.second;
+        // heavy groups first for load balance
+        std::stable_sort(A.groups.begin(), A.groups.end(),
+                         [](const std::pair<double, JKGroup> &x, const std::pair<double, JKGroup> &y) {
+                             return x.first > y.first;
                          });
-        ClassLaunch L = cstat[kv.first];
+        ClassLaunch L = A.stat;
         cls_ls(kv.first.first, L.la, L.lb);
         cls_ls(kv.first.second, L.lc, L.ld);
         L.first = (int)tasks.size();
-        for (auto &e : lst) tasks.push_back(std::get<1>(e));
+        L.gfirst = (int)groups.size();
+        for (auto &g : A.groups) {
+            JKGroup G = g.second;
+            const int src = G.first;
+            G.first = (int)tasks.size();
+            for (int t = 0; t < G.count; t++) tasks.push_back(A.tasks[src + t]);
+            groups.push_back(G);
+        }
         L.count = (int)tasks.size() - L.first;
+        L.gcount = (int)groups.size() - L.gfirst;
         if (L.count) launches_.push_back(L);
     }
     stats_.launches = (int)launches_.size();
@@ -359,6 +389,8 @@ void Context::build_tasks()
     for (auto &e : evc_) check(hipEventCreate(&e), "hipEventCreate");
     dev_free(d_tasks_);
     d_tasks_ = dev_upload(tasks);
+    dev_free(d_groups_);
+    d_groups_ = dev_upload(groups);
     dev_free(d_counters_);
     check(hipMalloc(&d_counters_, sizeof(int) * std::max<size_t>(launches_.size(), 1)), "hipMalloc");
 }
@@ -371,10 +403,22 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     const DevBasis B = devb();
     check(launch_zero_counters(d_counters_, (int)launches_.size(), st), "zero counters");
     check(hipEventRecord(ev0_, st), "hipEventRecord");
+    if (max_lds_ == 0) {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&max_lds_, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+    }
     for (size_t k = 0; k < launches_.size(); k++) {
         const ClassLaunch &L = launches_[k];
-        check(launch_jk(L.la, L.lb, L.lc, L.ld, d_tasks_ + L.first, L.count, d_counters_ + k, B, D_dev, d_M_, nao_, st),
-              "jk launch");
+        const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, nao_);
+        if (!no_group_ && lds <= (size_t)max_lds_)
+            check(launch_jk_group(L.la, L.lb, L.lc, L.ld, d_groups_ + L.gfirst, L.gcount, d_tasks_, d_counters_ + k, B,
+                                  D_dev, d_M_, nao_, st),
+                  "jk group launch");
+        else
+            check(launch_jk(L.la, L.lb, L.lc, L.ld, d_tasks_ + L.first, L.count, d_counters_ + k, B, D_dev, d_M_, nao_,
+                            st),
+                  "jk launch");
         check(hipEventRecord(evc_[k], st), "hipEventRecord");
     }
     check(hipEventRecord(ev1_, st), "hipEventRecord");

@@ -33,7 +33,8 @@ struct Stats {
 
 struct ClassLaunch {
     int la, lb, lc, ld;
-    int first, count;  // range in the device task array
+    int first, count;    // range in the device task array
+    int gfirst, gcount;  // range in the device group array
     long long quartets = 0, prim_quartets = 0, lanes = 0;
     double flops = 0;
     float ms = 0.f;    // device time of this launch in the last build
@@ -110,7 +111,10 @@ private:
     PrimPair *d_pp_ = nullptr;
     int *d_ao_off_ = nullptr;
     JKTask *d_tasks_ = nullptr;
+    JKGroup *d_groups_ = nullptr;
     int *d_counters_ = nullptr;  // one work counter per class launch
+    int max_lds_ = 0;
+    bool no_group_ = false;      // LIBRINT_NO_GROUP: flat per-task kernels only
     double *d_M_ = nullptr, *d_D_ = nullptr, *d_G_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<hipEvent_t> evc_;  // one event after each class launch

@@ -171,12 +171,24 @@ __device__ __forceinline__ double rcp_pos(double x)
     return fma(y, e, y);
 }
 
+// LDS stride (doubles) of one table segment: the (DEG+1)*NV coefficients
+// padded so that the stride is 16 bytes past a multiple of 128 bytes.  Lanes
+// of a wavefront read the same coefficient row of different segments; with
+// this stride consecutive segments start in different 16-byte bank groups
+// instead of all aliasing onto one group.
+__host__ __device__ constexpr int rys_seg_stride(int ncoef)
+{
+    return ncoef + ((2 - ncoef % 16) + 16) % 16;
+}
+
 // Rys nodes/weights from a table block of one root count staged in LDS:
-// rt = [seg][c][2N] (v then w), or for WONLY = [seg][c] weights only.
+// rt = [seg][c][2N] (v then w), or for WONLY = [seg][c] weights only, with
+// segments rys_seg_stride apart.
 template <int N, bool WONLY>
 __device__ __forceinline__ void rys_nodes_lds(double x, const double *rt, double *v, double *w)
 {
     constexpr int NV = WONLY ? 1 : 2 * N;
+    constexpr int STRIDE = rys_seg_stride((RYS_DEG + 1) * NV);
     if (x >= (double)RYS_NSEG[N]) {
         const double rs = rsqrt_pos(x);
         const double rx = rs * rs;
@@ -189,31 +201,16 @@ __device__ __forceinline__ void rys_nodes_lds(double x, const double *rt, double
     }
     const int k = (int)x;
     const double s = 2.0 * (x - (double)k) - 1.0;
-    const double s2 = 2.0 * s;
-    const double *c = rt + k * (RYS_DEG + 1) * NV;
-    double b1[NV], b2[NV];
-#pragma unroll
-    for (int j = 0; j < NV; j++) {
-        b1[j] = 0.0;
-        b2[j] = 0.0;
-    }
-#pragma unroll
-    for (int m = RYS_DEG; m >= 1; m--) {
-#pragma unroll
-        for (int j = 0; j < NV; j++) {
-            const double b0 = fma(s2, b1[j], c[m * NV + j] - b2[j]);
-            b2[j] = b1[j];
-            b1[j] = b0;
-        }
-    }
+    const double s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
+    const double *c = rt + k * STRIDE;
     if constexpr (WONLY) {
-        w[0] = fma(s, b1[0], c[0] - b2[0]);
+        w[0] = rys_estrin11(c, 1, s, s2, s4, s8);
         v[0] = 0.0;
     } else {
 #pragma unroll
         for (int j = 0; j < N; j++) {
-            v[j] = fma(s, b1[j], c[j] - b2[j]);
-            w[j] = fma(s, b1[N + j], c[N + j] - b2[N + j]);
+            v[j] = rys_estrin11(c + j, NV, s, s2, s4, s8);
+            w[j] = rys_estrin11(c + N + j, NV, s, s2, s4, s8);
         }
     }
 }
@@ -224,7 +221,9 @@ struct RysLds {
     static constexpr int N = C::NROOTS;
     static constexpr bool WONLY = (C::NMAX + C::MMAX) == 0;
     static constexpr int NV = WONLY ? 1 : 2 * N;
-    static constexpr int LEN = RYS_NSEG[N] * (RYS_DEG + 1) * NV;
+    static constexpr int NCOEF = (RYS_DEG + 1) * NV;       // per segment
+    static constexpr int STRIDE = rys_seg_stride(NCOEF);    // padded, in LDS
+    static constexpr int LEN = RYS_NSEG[N] * STRIDE;
 };
 
 // One primitive quartet, all roots, accumulated into eri (double precision).

@@ -13,14 +13,18 @@ libcint's CINTrys_roots (reference src/rys_roots.rs:3036-3064), which returns
 u = v/(1-v) and the plain weights w with sum_r w_r = F_0(x).
 
 Range [0, XCUT(n)) is split into unit-length segments; on each one every v_r and
-w_r is a degree-DEG Chebyshev series in the local variable s in [-1, 1].  Above
+w_r is a degree-DEG polynomial in the local variable s in [-1, 1], obtained as
+the Chebyshev interpolant at DEG+1 Chebyshev nodes and stored in the monomial
+basis of s (converted in 50-digit arithmetic) so that the device can evaluate
+it with Estrin's scheme.  Above
 XCUT(n) the asymptotic Gauss-Hermite limit is exact to double precision:
 v_r = h_r/x, w_r = H_r/sqrt(x), with h_r the squared positive nodes and H_r the
 matching weights of the 2n-point Gauss-Hermite rule (weight e^{-s^2}).
 
-Table layout (one block per n, GPU friendly: one lane runs all 2n Clenshaw
-recurrences of its segment with contiguous loads):
+Table layout (one block per n, GPU friendly: one lane evaluates all 2n
+polynomials of its segment with contiguous loads):
     RYS_CHEB[off(n) + (seg*(DEG+1) + c)*2n + j],  j < n: v_j, j >= n: w_{j-n}
+with c the power of s.
 
 Run:  python librint_amd/csrc/gen_rys_table.py   (writes rys_table.inc next to it)
 The output is committed; the generator needs mpmath and is not run at build time.
@@ -97,32 +101,50 @@ def cheb_fit(job):
     half = mp.mpf(SEGLEN) / 2
     nodes = [mp.cos(mp.pi * (j + mp.mpf(1) / 2) / (DEG + 1)) for j in range(DEG + 1)]
     vals = [rys_exact(n, c + half * t) for t in nodes]
-    out = []
-    for c_idx in range(DEG + 1):
-        row = []
-        for which in (0, 1):
-            for r in range(n):
-                f = [vals[j][which][r] for j in range(DEG + 1)]
-                s = 2 * mp.fsum(f[j] * mp.cos(mp.pi * c_idx * (j + mp.mpf(1) / 2) / (DEG + 1))
+    # Chebyshev polynomials T_0..T_DEG in the monomial basis (exact integers)
+    T = [[mp.mpf(1)] + [mp.mpf(0)] * DEG, [mp.mpf(0), mp.mpf(1)] + [mp.mpf(0)] * (DEG - 1)]
+    for m in range(2, DEG + 1):
+        T.append([2 * (T[m - 1][k - 1] if k > 0 else 0) - T[m - 2][k] for k in range(DEG + 1)])
+    cols = []
+    for which in (0, 1):
+        for r in range(n):
+            f = [vals[j][which][r] for j in range(DEG + 1)]
+            cheb = []
+            for c_idx in range(DEG + 1):
+                a = 2 * mp.fsum(f[j] * mp.cos(mp.pi * c_idx * (j + mp.mpf(1) / 2) / (DEG + 1))
                                 for j in range(DEG + 1)) / (DEG + 1)
-                if c_idx == 0:
-                    s /= 2
-                row.append(float(s))
-        out.append(row)
-    # validation at off-node points, evaluated with double Clenshaw
+                cheb.append(a / 2 if c_idx == 0 else a)
+            mono = [mp.fsum(cheb[m] * T[m][k] for m in range(DEG + 1)) for k in range(DEG + 1)]
+            cols.append([float(x) for x in mono])
+    out = [[cols[j][k] for j in range(2 * n)] for k in range(DEG + 1)]
+    # validation at off-node points, evaluated in double as the device does (Estrin)
     worst = [0.0, 0.0]
     for t in [-1.0, -0.77, -0.31, 0.13, 0.58, 0.91, 1.0]:
         ex = rys_exact(n, c + half * mp.mpf(t))
         for which in (0, 1):
             for r in range(n):
                 j = which * n + r
-                b1 = b2 = 0.0
-                for m in range(DEG, 0, -1):
-                    b0 = out[m][j] + 2 * t * b1 - b2
-                    b2, b1 = b1, b0
-                v = out[0][j] + t * b1 - b2
+                v = estrin([out[k][j] for k in range(DEG + 1)], t)
                 worst[which] = max(worst[which], abs(v - float(ex[which][r])))
     return n, seg, out, worst
+
+
+def _fma(a, b, c):
+    # exact-rounded a*b + c in double via mpmath (the device uses fma)
+    return float(mp.mpf(a) * mp.mpf(b) + mp.mpf(c))
+
+
+def estrin(a, s):
+    """Estrin evaluation of sum_k a[k] s^k (degree 11), the device's order."""
+    s2 = s * s
+    s4 = s2 * s2
+    s8 = s4 * s4
+    q = [_fma(a[2 * i + 1], s, a[2 * i]) for i in range(6)]
+    r0 = _fma(q[1], s2, q[0])
+    r1 = _fma(q[3], s2, q[2])
+    r2 = _fma(q[5], s2, q[4])
+    t0 = _fma(r1, s4, r0)
+    return _fma(r2, s8, t0)
 
 
 def pick_xcut(n, hh):
@@ -165,8 +187,9 @@ def main():
             worst[n][1] = max(worst[n][1], w[1])
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rys_table.inc")
     with open(path, "w") as f:
-        f.write("// Generated by gen_rys_table.py -- piecewise Chebyshev Rys nodes/weights.\n")
-        f.write("// Max abs error of double Clenshaw vs %d-digit Golub-Welsch, per n (v, w):\n" % DPS)
+        f.write("// Generated by gen_rys_table.py -- piecewise polynomial Rys nodes/weights\n")
+        f.write("// (Chebyshev interpolants stored in the monomial basis of the local variable).\n")
+        f.write("// Max abs error of double Estrin evaluation vs %d-digit Golub-Welsch, per n (v, w):\n" % DPS)
         for n in range(1, NMAX + 1):
             f.write("//   n=%d xcut=%d nseg=%d  %.2e  %.2e\n" % (n, xcut[n], nseg[n], worst[n][0], worst[n][1]))
         f.write("// Include after defining RYS_STORAGE (e.g. `static const` on the host,\n")

@@ -30,6 +30,14 @@ struct JKTask {
     int pad;
 };
 
+// All tasks of one bra pair within one class-pair launch, [first, first+count).
+struct JKGroup {
+    int bra;
+    int first;
+    int count;
+    int pad;
+};
+
 // One thread of the list kernel.
 struct ListTask {
     int bra;
@@ -42,6 +50,11 @@ bool class_supported(int la, int lb, int lc, int ld);
 hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, int *counter, const DevBasis &B,
                      const double *D, double *M, int nao, hipStream_t st);
 hipError_t launch_zero_counters(int *p, int n, hipStream_t st);
+// grouped form (exchange rows of the bra in LDS); jk_group_lds gives its LDS
+// bytes per workgroup, (size_t)-1 for classes without a grouped kernel
+size_t jk_group_lds(int la, int lb, int lc, int ld, int nao);
+hipError_t launch_jk_group(int la, int lb, int lc, int ld, const JKGroup *groups, int ngroup, const JKTask *tasks,
+                           int *counter, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st);
 hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
                        double *out, int mode, int nao, hipStream_t st);
 hipError_t launch_symmetrize(const double *M, double *G, int nao, hipStream_t st);

@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 #include "eri_core.h"
 #include "kernels.h"
@@ -206,7 +208,11 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
     __shared__ double rt[R::LEN];
     {
         const double *src = RYS_CHEB + RYS_OFF[R::N];
-        for (int i = threadIdx.x; i < R::LEN; i += blockDim.x) rt[i] = R::WONLY ? src[2 * i + 1] : src[i];
+        for (int i = threadIdx.x; i < R::LEN; i += blockDim.x) {
+            const int seg = i / R::STRIDE, c = i - seg * R::STRIDE;
+            const int g = seg * R::NCOEF + c;  // index in the unpadded (WONLY: weights-only) block
+            rt[i] = (c >= R::NCOEF) ? 0.0 : (R::WONLY ? src[2 * g + 1] : src[g]);
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -217,6 +223,206 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
         if (t >= ntask) break;
         jk_task<LA, LB, LC, LD>(tasks[t], B, D, M, nao, lane, rt);
     }
+}
+
+// ------------------------------------------------------ grouped J/K kernel --
+// One workgroup owns one bra shell pair (a,b) and all its kets of this class
+// pair (a "group" of tasks); its 4 wavefronts take the group's tasks in turn.
+// Every exchange contribution of the group lands in the bra's own AO rows
+// (M[a][*] and M[b][*]), so they are accumulated in LDS (ds_add_f64) and
+// flushed once per group with coalesced global atomics; the bra-bra Coulomb
+// block is reduced over the whole group.  Only the ket-ket Coulomb block
+// (M[c][d], one atomic per element per quartet) goes straight to memory.
+template <int LA, int LB, int LC, int LD>
+__device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &bra, const DevBasis &B,
+                                             const double *__restrict__ D, double *__restrict__ M, int nao, int lane,
+                                             const double *rt, double *Krow, double *mab)
+{
+    using C = Class<LA, LB, LC, LD>;
+    const bool active = lane < t.nket;
+    const int ketp = t.ket0 + (active ? lane : 0);
+    const PairView ket = load_pair(B, ketp);
+
+    double eri[C::NF];
+#pragma unroll
+    for (int f = 0; f < C::NF; f++) eri[f] = 0.0;
+    const double rbd[3] = {bra.rb[0] - ket.rb[0], bra.rb[1] - ket.rb[1], bra.rb[2] - ket.rb[2]};
+    for (int ik = 0; ik < ket.npp; ik++) {
+        const PrimPair q = B.pp[ket.off + ik];
+        for (int ib = 0; ib < bra.npp; ib++) {
+            const PrimPair p = B.pp[bra.off + ib];
+            prim_quartet_fp64<LA, LB, LC, LD, true>(eri, p, q, rbd, bra.rab, ket.rab, rt);
+        }
+    }
+    if (!active) return;
+    double s = 1.0;
+    if (bra.sa == bra.sb) s *= 0.5;
+    if (ket.sa == ket.sb) s *= 0.5;
+    if (t.bra == ketp) s *= 0.5;
+    const size_t n = (size_t)nao;
+    // bra-bra Coulomb: per-lane partial sums, reduced at the end of the group
+#pragma unroll
+    for (int fa = 0; fa < C::NFA; fa++)
+#pragma unroll
+        for (int fb = 0; fb < C::NFB; fb++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int fd = 0; fd < C::NFD; fd++)
+#pragma unroll
+                for (int fc = 0; fc < C::NFC; fc++)
+                    acc += D[(ket.aoa + fc) * n + ket.aob + fd] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+            mab[fa * C::NFB + fb] += 2.0 * s * acc;
+        }
+    // ket-ket Coulomb
+#pragma unroll
+    for (int fc = 0; fc < C::NFC; fc++)
+#pragma unroll
+        for (int fd = 0; fd < C::NFD; fd++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int fb = 0; fb < C::NFB; fb++)
+#pragma unroll
+                for (int fa = 0; fa < C::NFA; fa++)
+                    acc += D[(bra.aoa + fa) * n + bra.aob + fb] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+            atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], 2.0 * s * acc);
+        }
+    const double h = -0.5 * s;
+    double *Ka = Krow, *Kb = Krow + C::NFA * n;
+#pragma unroll
+    for (int fc = 0; fc < C::NFC; fc++) {
+#pragma unroll
+        for (int fa = 0; fa < C::NFA; fa++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int fd = 0; fd < C::NFD; fd++)
+#pragma unroll
+                for (int fb = 0; fb < C::NFB; fb++)
+                    acc += D[(bra.aob + fb) * n + ket.aob + fd] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+            atomicAdd(&Ka[fa * n + ket.aoa + fc], h * acc);
+        }
+#pragma unroll
+        for (int fb = 0; fb < C::NFB; fb++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int fd = 0; fd < C::NFD; fd++)
+#pragma unroll
+                for (int fa = 0; fa < C::NFA; fa++)
+                    acc += D[(bra.aoa + fa) * n + ket.aob + fd] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+            atomicAdd(&Kb[fb * n + ket.aoa + fc], h * acc);
+        }
+    }
+#pragma unroll
+    for (int fd = 0; fd < C::NFD; fd++) {
+#pragma unroll
+        for (int fa = 0; fa < C::NFA; fa++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int fc = 0; fc < C::NFC; fc++)
+#pragma unroll
+                for (int fb = 0; fb < C::NFB; fb++)
+                    acc += D[(bra.aob + fb) * n + ket.aoa + fc] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+            atomicAdd(&Ka[fa * n + ket.aob + fd], h * acc);
+        }
+#pragma unroll
+        for (int fb = 0; fb < C::NFB; fb++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int fc = 0; fc < C::NFC; fc++)
+#pragma unroll
+                for (int fa = 0; fa < C::NFA; fa++)
+                    acc += D[(bra.aoa + fa) * n + ket.aoa + fc] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+            atomicAdd(&Kb[fb * n + ket.aob + fd], h * acc);
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+__global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
+                                                       const JKTask *__restrict__ tasks, int *__restrict__ counter,
+                                                       DevBasis B, const double *__restrict__ D,
+                                                       double *__restrict__ M, int nao)
+{
+    using C = Class<LA, LB, LC, LD>;
+    using R = RysLds<LA, LB, LC, LD>;
+    extern __shared__ double lds[];
+    double *rt = lds;                       // Rys table block
+    double *mab_l = lds + R::LEN;           // bra-bra block
+    double *Krow = mab_l + C::NFA * C::NFB; // (NFA + NFB) rows of nao
+    __shared__ int g_s;
+    {
+        const double *src = RYS_CHEB + RYS_OFF[R::N];
+        for (int i = threadIdx.x; i < R::LEN; i += blockDim.x) {
+            const int seg = i / R::STRIDE, c = i - seg * R::STRIDE;
+            const int g = seg * R::NCOEF + c;
+            rt[i] = (c >= R::NCOEF) ? 0.0 : (R::WONLY ? src[2 * g + 1] : src[g]);
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nrow = (C::NFA + C::NFB) * nao;
+    const size_t n = (size_t)nao;
+    for (;;) {
+        if (threadIdx.x == 0) g_s = atomicAdd(counter, 1);
+        for (int i = threadIdx.x; i < nrow; i += blockDim.x) Krow[i] = 0.0;
+        if (threadIdx.x < C::NFA * C::NFB) mab_l[threadIdx.x] = 0.0;
+        __syncthreads();
+        const int g = g_s;
+        if (g >= ngroup) break;
+        const JKGroup G = groups[g];
+        const PairView bra = load_pair(B, G.bra);
+        double mab[C::NFA * C::NFB];
+#pragma unroll
+        for (int e = 0; e < C::NFA * C::NFB; e++) mab[e] = 0.0;
+        for (int t = G.first + wave; t < G.first + G.count; t += 4)
+            jk_task_rows<LA, LB, LC, LD>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab);
+#pragma unroll
+        for (int e = 0; e < C::NFA * C::NFB; e++) {
+            const double v = wave_sum(mab[e]);
+            if (lane == 0) atomicAdd(&mab_l[e], v);
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+            const double v = Krow[i];
+            if (v != 0.0) {
+                const int r = i / nao, col = i - r * nao;
+                const size_t row = (r < C::NFA) ? (size_t)(bra.aoa + r) : (size_t)(bra.aob + r - C::NFA);
+                atomicAdd(&M[row * n + col], v);
+            }
+        }
+        if (threadIdx.x < C::NFA * C::NFB)
+            atomicAdd(&M[(bra.aoa + threadIdx.x / C::NFB) * n + bra.aob + threadIdx.x % C::NFB], mab_l[threadIdx.x]);
+        __syncthreads();
+    }
+}
+
+template <int LA, int LB, int LC, int LD>
+static size_t group_lds_bytes(int nao)
+{
+    using C = Class<LA, LB, LC, LD>;
+    using R = RysLds<LA, LB, LC, LD>;
+    return sizeof(double) * ((size_t)R::LEN + C::NFA * C::NFB + (size_t)(C::NFA + C::NFB) * nao);
+}
+
+template <int LA, int LB, int LC, int LD>
+static hipError_t launch_jk_group_t(const JKGroup *groups, int ngroup, const JKTask *tasks, int *counter,
+                                    const DevBasis &B, const double *D, double *M, int nao, hipStream_t st)
+{
+    const size_t lds = group_lds_bytes<LA, LB, LC, LD>(nao);
+    static int grid = 0, grid_nao = -1;
+    if (grid == 0 || grid_nao != nao) {
+        int dev = 0, ncu = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_group_kernel<LA, LB, LC, LD>, 256, lds);
+        grid = std::max(1, ncu) * std::max(1, per);
+        grid_nao = nao;
+        if (getenv("LIBRINT_DEBUG"))
+            fprintf(stderr, "librint: jk_group<%d%d|%d%d> lds %zu blocks/cu %d grid %d\n", LA, LB, LC, LD, lds, per,
+                    grid);
+    }
+    const int g = std::min(grid, ngroup);
+    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks, counter, B,
+                       D, M, nao);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------ team J/K kernel --
@@ -575,7 +781,11 @@ static hipError_t launch_jk_persistent(const JKTask *tasks, int ntask, int *coun
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_kernel<LA, LB, LC, LD>, 256, 0);
-        grid = std::max(1, ncu) * std::max(1, per);
+        int mult = 1;
+        if (const char *e = getenv("LIBRINT_JK_GRID_MULT")) mult = std::max(1, atoi(e));
+        grid = std::max(1, ncu) * std::max(1, per) * mult;
+        if (getenv("LIBRINT_DEBUG"))
+            fprintf(stderr, "librint: jk<%d%d|%d%d> ncu %d blocks/cu %d grid %d\n", LA, LB, LC, LD, ncu, per, grid);
     }
     const int g = std::min(grid, (ntask + 3) / 4);
     hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD>), dim3(g), dim3(256), 0, st, tasks, ntask, counter, B, D, M, nao);
@@ -609,6 +819,27 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int nt
         return hipGetLastError();                                                                                      \
     }
     LR_TEAM_CLASSES(X)
+#undef X
+    return hipErrorInvalidValue;
+}
+
+size_t jk_group_lds(int la, int lb, int lc, int ld, int nao)
+{
+#define X(a, b, c, d)                                                                                                  \
+    if (la == a && lb == b && lc == c && ld == d) return group_lds_bytes<a, b, c, d>(nao);
+    LR_REG_CLASSES(X)
+#undef X
+    return (size_t)-1;
+}
+
+hipError_t launch_jk_group(int la, int lb, int lc, int ld, const JKGroup *groups, int ngroup, const JKTask *tasks,
+                           int *counter, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st)
+{
+    if (ngroup == 0) return hipSuccess;
+#define X(a, b, c, d)                                                                                                  \
+    if (la == a && lb == b && lc == c && ld == d)                                                                      \
+        return launch_jk_group_t<a, b, c, d>(groups, ngroup, tasks, counter, B, D, M, nao, st);
+    LR_REG_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
 }

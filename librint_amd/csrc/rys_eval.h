@@ -7,8 +7,9 @@
 // sum_r w_r = F_0(x).
 //
 // x < RYS_XCUT[n]: segment k = floor(x) of unit length, local variable
-//   s = 2(x - k) - 1 in [-1, 1], 2n Clenshaw recurrences of degree RYS_DEG
-//   over contiguous coefficients RYS_CHEB[off + (k*(DEG+1) + c)*2n + j].
+//   s = 2(x - k) - 1 in [-1, 1], 2n polynomials of degree RYS_DEG = 11 in s
+//   (monomial coefficients RYS_CHEB[off + (k*(DEG+1) + c)*2n + j], c the
+//   power) evaluated with Estrin's scheme.
 // x >= RYS_XCUT[n]: Gauss-Hermite limit v_r = h_r / x, w_r = H_r / sqrt(x).
 #pragma once
 
@@ -29,6 +30,22 @@ namespace librint {
 #endif
 #include "rys_table.inc"
 #undef RYS_STORAGE
+
+// Estrin evaluation of a degree-11 polynomial sum_c a[c*stride] s^c
+// (depth 4 instead of the 11 dependent steps of Horner's rule).
+RYS_HD inline double rys_estrin11(const double *a, int stride, double s, double s2, double s4, double s8)
+{
+    const double q0 = fma(a[1 * stride], s, a[0]);
+    const double q1 = fma(a[3 * stride], s, a[2 * stride]);
+    const double q2 = fma(a[5 * stride], s, a[4 * stride]);
+    const double q3 = fma(a[7 * stride], s, a[6 * stride]);
+    const double q4 = fma(a[9 * stride], s, a[8 * stride]);
+    const double q5 = fma(a[11 * stride], s, a[10 * stride]);
+    const double r0 = fma(q1, s2, q0);
+    const double r1 = fma(q3, s2, q2);
+    const double r2 = fma(q5, s2, q4);
+    return fma(r2, s8, fma(r1, s4, r0));
+}
 
 template <int N>
 RYS_HD inline void rys_nodes(double x, double *v, double *w)
@@ -54,29 +71,15 @@ RYS_HD inline void rys_nodes(double x, double *v, double *w)
         }
         return;
     }
+    static_assert(RYS_DEG == 11, "Estrin evaluation is written for degree 11");
     const int k = (int)x;
     const double s = 2.0 * (x - (double)k) - 1.0;
-    const double s2 = 2.0 * s;
+    const double s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
     const double *c = RYS_CHEB + RYS_OFF[N] + k * (RYS_DEG + 1) * NV;
-    double b1[NV], b2[NV];
-#pragma unroll
-    for (int j = 0; j < NV; j++) {
-        b1[j] = 0.0;
-        b2[j] = 0.0;
-    }
-#pragma unroll
-    for (int m = RYS_DEG; m >= 1; m--) {
-#pragma unroll
-        for (int j = 0; j < NV; j++) {
-            const double b0 = fma(s2, b1[j], c[m * NV + j] - b2[j]);
-            b2[j] = b1[j];
-            b1[j] = b0;
-        }
-    }
 #pragma unroll
     for (int j = 0; j < N; j++) {
-        v[j] = fma(s, b1[j], c[j] - b2[j]);
-        w[j] = fma(s, b1[N + j], c[N + j] - b2[N + j]);
+        v[j] = rys_estrin11(c + j, NV, s, s2, s4, s8);
+        w[j] = rys_estrin11(c + N + j, NV, s, s2, s4, s8);
     }
 }
 

@@ -37,12 +37,10 @@ def table_nodes(n, x):
     s = 2 * (x - k) - 1
     nv = 2 * n
     c = cheb[off[n] + k * (deg + 1) * nv: off[n] + (k + 1) * (deg + 1) * nv].reshape(deg + 1, nv)
-    b1 = np.zeros(nv)
-    b2 = np.zeros(nv)
-    for m in range(deg, 0, -1):
-        b0 = 2 * s * b1 + c[m] - b2
-        b2, b1 = b1, b0
-    val = s * b1 + c[0] - b2
+    # monomial coefficients in s (Horner here; the device uses Estrin)
+    val = np.zeros(nv)
+    for m in range(deg, -1, -1):
+        val = val * s + c[m]
     return val[:n], val[n:]
 
 
