@@ -300,7 +300,8 @@ void Context::build_tasks()
     stats_.candidates = np * (np + 1) / 2;
     // A group = all tasks of one bra pair for one ket class (capped so that a
     // heavy bra is spread over a few workgroups).
-    const int GROUP_MAX_TASKS = 32;
+    int GROUP_MAX_TASKS = 8;
+    if (const char *e = getenv("LIBRINT_GROUP_TASKS")) GROUP_MAX_TASKS = std::max(1, atoi(e));
     struct Acc {
         std::vector<JKTask> tasks;
         std::vector<std::pair<double, JKGroup>> groups;
