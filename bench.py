@@ -46,6 +46,8 @@ def workload(name):
         return mol.h2o_sto3g(), 0.0
     if name.startswith("alkane"):
         return mol.alkane_631g(int(name[6:])), 1e-12
+    if name.startswith("w") and name.endswith("_et"):
+        return mol.water_cluster(int(name[1:-3])), 1e-12
     raise SystemExit("unknown workload " + name)
 
 
@@ -77,6 +79,9 @@ def main():
     ap.add_argument("--workload", default="c16h34_631g")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--tau-mix", type=float, default=1e-6)
+    ap.add_argument("--tau-skip", type=float, default=0.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the Fock build is sharded by quartet bins "
@@ -89,6 +94,11 @@ def main():
     m, tau = workload(args.workload)
     P = core_guess_density(m)
     ctx = FockContext(m, schwarz_tol=tau)
+    mixed = args.precision == "mixed"
+    G64 = None
+    if mixed or args.tau_skip > 0:
+        G64 = ctx.fock(P)  # FP64 reference build (untimed) for the error report
+        ctx.set_precision(1 if mixed else 0, args.tau_mix if mixed else 0.0, args.tau_skip)
     dev = torch.device("cuda:0")
     Pd = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
     Gd = torch.empty_like(Pd)
@@ -107,6 +117,15 @@ def main():
         ctx.fock(P)
         ev_ms.append(ctx.stats()["kernel_ms"])
     st = ctx.stats()
+    err = None
+    if G64 is not None:
+        Gm = ctx.fock(P)
+        fc = ctx.flag_counts()
+        err = {"tau_mix": args.tau_mix if mixed else None, "tau_skip": args.tau_skip,
+               "energy_error_Eh": float(0.5 * np.sum(P * (Gm - G64))),
+               "max_abs_dG": float(np.abs(Gm - G64).max()),
+               "quartets_fp64_fp32_skipped": fc["quartets"],
+               "prim_quartets_fp64_fp32_skipped": fc["prim_quartets"]}
     classes = [{"class": "(%d%d|%d%d)" % (int(c["la"]), int(c["lb"]), int(c["lc"]), int(c["ld"])),
                 "quartets": int(c["quartets"]), "prim_quartets": int(c["prim_quartets"]),
                 "lane_util": c["quartets"] / max(c["lanes"], 1), "ms": c["ms"],
@@ -139,6 +158,11 @@ def main():
                      "kernel_ms": jk_ms, "flops_per_launch": st["flops"]},
         "classes": classes,
     }
+    if err is not None:
+        line["metric"] = "RHF Fock-build s/iter + ERI quartets/s (%s fused J/K)" % ("mixed FP32/FP64" if mixed else "FP64, density screened")
+        line["dtype"] = "f32+f64" if mixed else "f64"
+        line["config"]["precision"] = args.precision
+        line["mixed"] = err
     if not args.no_cpu:
         rate, t_full, nq, used = cpu_baseline(m, P, args.cpu_seconds)
         line["cpu_baseline"] = {

@@ -83,6 +83,19 @@ int librint_ctx_nao(const librint_ctx *ctx);
 /* Set the Schwarz threshold (default 1e-14); returns 0 on success. */
 int librint_ctx_set_schwarz(librint_ctx *ctx, double tol);
 
+/* Precision of the Fock build.  mode 0: FP64 everywhere (default).
+ * mode 1: mixed -- quartets with Q_ab*Q_cd*dmax < tau_mix run their Rys
+ * recurrences in FP32 (the Rys table too; J/K accumulation stays FP64).  The
+ * split is fixed when the quartet lists are (re)built; dmax <= 0 takes max|D|
+ * of the next density.  tau_skip > 0 additionally drops, per build, the 64-ket
+ * tasks whose largest bound Q_ab*Q_cd*max(|D| over the six blocks a quartet
+ * reads) is below tau_skip (density screening; also honoured in mode 0).
+ * Returns 0 on success. */
+int librint_ctx_set_precision(librint_ctx *ctx, int mode, double tau_mix, double tau_skip, double dmax);
+/* Quartets and primitive quartets of the last build by task flag:
+ * out[0..2] quartets FP64/FP32/skipped, out[3..5] primitive quartets. */
+int librint_ctx_flag_counts(librint_ctx *ctx, double *out6);
+
 /* G = J - K/2 from host buffers (nao*nao each).  Returns 0 on success. */
 int librint_fock(librint_ctx *ctx, const double *P, double *G);
 /* Same on device buffers, enqueued on `stream` (a hipStream_t, NULL = default).

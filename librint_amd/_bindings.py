@@ -62,6 +62,11 @@ library.librint_ctx_stats.argtypes = (ctypes.c_void_p, _dp, ctypes.c_int)
 library.librint_ctx_stats.restype = ctypes.c_int
 library.librint_ctx_class_stats.argtypes = (ctypes.c_void_p, _dp, ctypes.c_int)
 library.librint_ctx_class_stats.restype = ctypes.c_int
+library.librint_ctx_set_precision.argtypes = (ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double)
+library.librint_ctx_set_precision.restype = ctypes.c_int
+library.librint_ctx_flag_counts.argtypes = (ctypes.c_void_p, _dp)
+library.librint_ctx_flag_counts.restype = ctypes.c_int
 library.librint_scf.argtypes = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_int, _dp, _ip)
 library.librint_scf.restype = ctypes.c_double
@@ -74,6 +79,6 @@ EXPORTED = (
     "dHcore_c", "dR_c", "danalytical_c", "denergy_c", "fock_c", "librint_ctx_create",
     "librint_ctx_destroy", "librint_ctx_nao", "librint_ctx_set_schwarz", "librint_fock",
     "librint_fock_device", "librint_int2e_shells", "librint_schwarz", "librint_ctx_stats",
-    "librint_ctx_class_stats",
+    "librint_ctx_class_stats", "librint_ctx_set_precision", "librint_ctx_flag_counts",
     "librint_scf", "librint_last_error",
 )

@@ -34,6 +34,16 @@ class FockContext:
         if library.librint_ctx_set_schwarz(self.h, float(tol)) != 0:
             raise RuntimeError(utils.last_error())
 
+    def set_precision(self, mode, tau_mix=0.0, tau_skip=0.0, dmax=0.0):
+        """mode 0 FP64; mode 1 mixed (FP32 recurrences for Q Q dmax < tau_mix)."""
+        if library.librint_ctx_set_precision(self.h, int(mode), float(tau_mix), float(tau_skip), float(dmax)) != 0:
+            raise RuntimeError(utils.last_error())
+
+    def flag_counts(self):
+        out = np.zeros(6)
+        library.librint_ctx_flag_counts(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return {"quartets": out[:3].astype(int).tolist(), "prim_quartets": out[3:].astype(int).tolist()}
+
     def fock(self, P):
         P = np.ascontiguousarray(P, dtype=np.float64)
         assert P.shape == (self.nao, self.nao)

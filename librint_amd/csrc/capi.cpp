@@ -222,6 +222,41 @@ int librint_ctx_set_schwarz(librint_ctx *ctx, double tol)
     }
 }
 
+int librint_ctx_set_precision(librint_ctx *ctx, int mode, double tau_mix, double tau_skip, double dmax)
+{
+    try {
+        if (!ctx) {
+            set_err("null context");
+            return -1;
+        }
+        ctx->ctx->set_precision(mode, tau_mix, tau_skip, dmax);
+        return 0;
+    } catch (std::exception &e) {
+        set_err(e.what());
+        return -1;
+    }
+}
+
+int librint_ctx_flag_counts(librint_ctx *ctx, double *out6)
+{
+    try {
+        if (!ctx || !out6) {
+            set_err("null argument");
+            return -1;
+        }
+        long long q[3], p[3];
+        ctx->ctx->flag_counts(q, p);
+        for (int i = 0; i < 3; i++) {
+            out6[i] = (double)q[i];
+            out6[3 + i] = (double)p[i];
+        }
+        return 0;
+    } catch (std::exception &e) {
+        set_err(e.what());
+        return -1;
+    }
+}
+
 int librint_fock(librint_ctx *ctx, const double *P, double *G)
 {
     try {

@@ -105,6 +105,12 @@ Context::~Context()
     dev_free(d_ao_off_);
     dev_free(d_tasks_);
     dev_free(d_groups_);
+    dev_free(d_task_group_);
+    dev_free(d_flags_);
+    dev_free(d_gmask_w_);
+    dev_free(d_gmask_);
+    dev_free(d_dsh_);
+    dev_free(d_dmax_);
     dev_free(d_counters_);
     dev_free(d_M_);
     dev_free(d_D_);
@@ -198,8 +204,9 @@ void Context::upload()
         rb[p] = make_double4(Bs.r[0], Bs.r[1], Bs.r[2], P.Q);
         rab[p] = make_double4(Bs.r[0] - A.r[0], Bs.r[1] - A.r[1], Bs.r[2] - A.r[2], 0.0);
     }
-    std::vector<int> aoo(shells_.size());
+    std::vector<int> aoo(shells_.size() + 1);
     for (size_t s = 0; s < shells_.size(); s++) aoo[s] = shells_[s].ao;
+    aoo[shells_.size()] = nao_;
     d_pair_info_ = dev_upload(info);
     d_pair_rb_ = dev_upload(rb);
     d_pair_rab_ = dev_upload(rab);
@@ -307,7 +314,8 @@ void Context::build_tasks()
         std::vector<std::pair<double, JKGroup>> groups;
         ClassLaunch stat;
     };
-    std::map<std::pair<int, int>, Acc> acc;
+    std::map<std::tuple<int, int, int>, Acc> acc;  // (bra class, ket class, fp32)
+    const bool mixed = precision_ == 1 && dmax_ > 0.0 && tau_mix_ > 0.0;
     for (int X = 0; X < nb; X++) {
         const Bin &bx = bins_[X];
         int la, lb;
@@ -315,7 +323,7 @@ void Context::build_tasks()
         for (int i = 0; i < bx.count; i++) {
             const int p = bx.first + i;
             const double Qp = pairs_[p].Q;
-            std::map<int, int> open;  // ket class -> index of the open group
+            std::map<std::pair<int, int>, int> open;  // (ket class, fp32) -> index of the open group
             for (int Y = 0; Y <= X; Y++) {
                 const Bin &by = bins_[Y];
                 int lc, ld;
@@ -334,33 +342,51 @@ void Context::build_tasks()
                 if (n <= 0) continue;
                 const double fp = flops_prim(la, lb, lc, ld);
                 const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-                const double fl = (double)n * bx.npp * by.npp * fp + 12.0 * nf * n;
-                stats_.quartets += n;
-                stats_.prim_quartets += (long long)n * bx.npp * by.npp;
-                stats_.integrals += (long long)n * nf;
-                stats_.flops += fl;
-                Acc &A = acc[{bx.cls, by.cls}];
-                A.stat.quartets += n;
-                A.stat.prim_quartets += (long long)n * bx.npp * by.npp;
-                A.stat.flops += fl;
-                A.stat.lanes += 64LL * ((n + 63) / 64);
-                for (int k0 = 0; k0 < n; k0 += 64) {
-                    auto it = open.find(by.cls);
-                    if (it == open.end() || A.groups[it->second].second.count >= GROUP_MAX_TASKS) {
-                        A.groups.push_back({0.0, JKGroup{p, (int)A.tasks.size(), 0, 0}});
-                        open[by.cls] = (int)A.groups.size() - 1;
-                        it = open.find(by.cls);
+                // mixed: kets [0, n64) keep FP64, [n64, n) have Q_b Q_k dmax < tau_mix
+                int n64 = n;
+                if (mixed && class_has_fp32(la, lb, lc, ld)) {
+                    int l2 = 0, h2 = n;
+                    while (l2 < h2) {
+                        const int mid = (l2 + h2) / 2;
+                        if (Qp * pairs_[by.first + mid].Q * dmax_ >= tau_mix_) l2 = mid + 1;
+                        else h2 = mid;
                     }
-                    const int nk = std::min(64, n - k0);
-                    A.tasks.push_back(JKTask{p, by.first + k0, nk, 0});
-                    A.groups[it->second].second.count++;
-                    A.groups[it->second].first += (double)nk * bx.npp * by.npp;
+                    n64 = l2;
+                }
+                for (int part = 0; part < 2; part++) {
+                    const int k_begin = part ? n64 : 0, k_end = part ? n : n64;
+                    const int nn = k_end - k_begin;
+                    if (nn <= 0) continue;
+                    const double fl = (double)nn * bx.npp * by.npp * fp + 12.0 * nf * nn;
+                    stats_.quartets += nn;
+                    stats_.prim_quartets += (long long)nn * bx.npp * by.npp;
+                    stats_.integrals += (long long)nn * nf;
+                    stats_.flops += fl;
+                    Acc &A = acc[std::make_tuple(bx.cls, by.cls, part)];
+                    A.stat.quartets += nn;
+                    A.stat.prim_quartets += (long long)nn * bx.npp * by.npp;
+                    A.stat.flops += fl;
+                    A.stat.lanes += 64LL * ((nn + 63) / 64);
+                    for (int k0 = k_begin; k0 < k_end; k0 += 64) {
+                        const std::pair<int, int> key{by.cls, part};
+                        auto it = open.find(key);
+                        if (it == open.end() || A.groups[it->second].second.count >= GROUP_MAX_TASKS) {
+                            A.groups.push_back({0.0, JKGroup{p, (int)A.tasks.size(), 0, 0}});
+                            open[key] = (int)A.groups.size() - 1;
+                            it = open.find(key);
+                        }
+                        const int nk = std::min(64, k_end - k0);
+                        A.tasks.push_back(JKTask{p, by.first + k0, nk, 0});
+                        A.groups[it->second].second.count++;
+                        A.groups[it->second].first += (double)nk * bx.npp * by.npp;
+                    }
                 }
             }
         }
     }
     std::vector<JKTask> tasks;
     std::vector<JKGroup> groups;
+    std::vector<int> task_group;
     for (auto &kv : acc) {
         Acc &A = kv.second;
         // heavy groups first for load balance
@@ -369,15 +395,19 @@ void Context::build_tasks()
                              return x.first > y.first;
                          });
         ClassLaunch L = A.stat;
-        cls_ls(kv.first.first, L.la, L.lb);
-        cls_ls(kv.first.second, L.lc, L.ld);
+        cls_ls(std::get<0>(kv.first), L.la, L.lb);
+        cls_ls(std::get<1>(kv.first), L.lc, L.ld);
+        L.fp32 = std::get<2>(kv.first);
         L.first = (int)tasks.size();
         L.gfirst = (int)groups.size();
         for (auto &g : A.groups) {
             JKGroup G = g.second;
             const int src = G.first;
             G.first = (int)tasks.size();
-            for (int t = 0; t < G.count; t++) tasks.push_back(A.tasks[src + t]);
+            for (int t = 0; t < G.count; t++) {
+                tasks.push_back(A.tasks[src + t]);
+                task_group.push_back((int)groups.size());
+            }
             groups.push_back(G);
         }
         L.count = (int)tasks.size() - L.first;
@@ -392,8 +422,46 @@ void Context::build_tasks()
     d_tasks_ = dev_upload(tasks);
     dev_free(d_groups_);
     d_groups_ = dev_upload(groups);
+    dev_free(d_task_group_);
+    d_task_group_ = dev_upload(task_group);
+    dev_free(d_flags_);
+    dev_free(d_gmask_w_);
+    dev_free(d_gmask_);
+    check(hipMalloc(&d_flags_, std::max<size_t>(tasks.size(), 1)), "hipMalloc");
+    check(hipMalloc(&d_gmask_w_, sizeof(unsigned int) * std::max<size_t>(groups.size(), 1)), "hipMalloc");
+    check(hipMalloc(&d_gmask_, std::max<size_t>(groups.size(), 1)), "hipMalloc");
+    ngroups_ = (int)groups.size();
+    h_tasks_ = tasks;
+    classified_ = false;
     dev_free(d_counters_);
-    check(hipMalloc(&d_counters_, sizeof(int) * std::max<size_t>(launches_.size(), 1)), "hipMalloc");
+    check(hipMalloc(&d_counters_, sizeof(int) * 2 * std::max<size_t>(launches_.size(), 1)), "hipMalloc");
+}
+
+void Context::set_precision(int mode, double tau_mix, double tau_skip, double dmax)
+{
+    if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
+    if (!(tau_mix >= 0) || !(tau_skip >= 0) || !(dmax >= 0)) throw std::invalid_argument("thresholds must be >= 0");
+    const bool rebuild = mode != precision_ || tau_mix != tau_mix_ || dmax != dmax_;
+    precision_ = mode;
+    tau_mix_ = tau_mix;
+    tau_skip_ = tau_skip;
+    dmax_ = dmax;
+    if (rebuild) build_tasks();
+}
+
+void Context::flag_counts(long long *q3, long long *p3)
+{
+    for (int f = 0; f < 3; f++) q3[f] = p3[f] = 0;
+    std::vector<unsigned char> fl(h_tasks_.size(), 0);
+    if (classified_ && !fl.empty())
+        check(hipMemcpy(fl.data(), d_flags_, fl.size(), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    for (const ClassLaunch &L : launches_)
+        for (int t = L.first; t < L.first + L.count; t++) {
+            const JKTask &T = h_tasks_[t];
+            const int f = fl[t] == 2 ? 2 : L.fp32;
+            q3[f] += T.nket;
+            p3[f] += (long long)T.nket * pairs_[T.bra].npp * pairs_[T.ket0].npp;
+        }
 }
 
 void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
@@ -402,23 +470,46 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
     check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
     const DevBasis B = devb();
-    check(launch_zero_counters(d_counters_, (int)launches_.size(), st), "zero counters");
-    check(hipEventRecord(ev0_, st), "hipEventRecord");
+    check(launch_zero_counters(d_counters_, 2 * (int)launches_.size(), st), "zero counters");
     if (max_lds_ == 0) {
         int dev = 0;
         check(hipGetDevice(&dev), "hipGetDevice");
         check(hipDeviceGetAttribute(&max_lds_, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+        if (getenv("LIBRINT_DEBUG")) fprintf(stderr, "librint: max LDS per block %d\n", max_lds_);
     }
+    if (precision_ == 1 && tau_mix_ > 0.0 && dmax_ <= 0.0) {
+        // split threshold from this density: max|D|, then rebuild the lists once
+        if (!d_dmax_) check(hipMalloc(&d_dmax_, sizeof(double)), "hipMalloc");
+        check(launch_absmax(D_dev, nn, d_dmax_, st), "absmax launch");
+        double dm = 0.0;
+        check(hipMemcpyAsync(&dm, d_dmax_, sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        dmax_ = dm > 0.0 ? dm : 1.0;
+        build_tasks();
+    }
+    const bool classify = tau_skip_ > 0.0;
+    const int nsh = (int)shells_.size();
+    if (classify) {
+        if (!d_dsh_) check(hipMalloc(&d_dsh_, sizeof(double) * (size_t)nsh * nsh), "hipMalloc");
+        check(launch_classify(d_tasks_, (int)h_tasks_.size(), d_task_group_, ngroups_, B, D_dev, d_ao_off_, nsh, nao_,
+                              d_dsh_, 0.0, tau_skip_, d_flags_, d_gmask_w_, d_gmask_, st),
+              "classify launch");
+    }
+    classified_ = classify;
+    const unsigned char *flags = classify ? d_flags_ : nullptr;
+    const unsigned char *gmask = classify ? d_gmask_ : nullptr;
+    check(hipEventRecord(ev0_, st), "hipEventRecord");
     for (size_t k = 0; k < launches_.size(); k++) {
         const ClassLaunch &L = launches_[k];
-        const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, nao_);
+        int *ctr = d_counters_ + 2 * k;
+        const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_);
         if (!no_group_ && lds <= (size_t)max_lds_)
-            check(launch_jk_group(L.la, L.lb, L.lc, L.ld, d_groups_ + L.gfirst, L.gcount, d_tasks_, d_counters_ + k, B,
-                                  D_dev, d_M_, nao_, st),
+            check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
+                                  gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, st),
                   "jk group launch");
         else
-            check(launch_jk(L.la, L.lb, L.lc, L.ld, d_tasks_ + L.first, L.count, d_counters_ + k, B, D_dev, d_M_, nao_,
-                            st),
+            check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + L.first, L.count, ctr,
+                            flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_, nao_, st),
                   "jk launch");
         check(hipEventRecord(evc_[k], st), "hipEventRecord");
     }

@@ -35,6 +35,7 @@ struct ClassLaunch {
     int la, lb, lc, ld;
     int first, count;    // range in the device task array
     int gfirst, gcount;  // range in the device group array
+    int fp32 = 0;        // FP32 recurrences for every task of this launch
     long long quartets = 0, prim_quartets = 0, lanes = 0;
     double flops = 0;
     float ms = 0.f;    // device time of this launch in the last build
@@ -52,6 +53,15 @@ public:
     // Screening threshold on Q_ij * Q_kl; rebuilds the task lists.
     void set_schwarz(double tau);
     double schwarz() const { return tau_; }
+    // Precision mode 0: FP64 everywhere.  Mode 1 (mixed): quartets with
+    // Q_ab Q_cd dmax < tau_mix use FP32 recurrences, the split being made
+    // once when the task lists are built (dmax <= 0: max|D| of the next
+    // density).  tau_skip > 0 drops, per build, the tasks whose
+    // density-weighted bound Q_ab Q_cd max(|D| over the six blocks) is below it.
+    void set_precision(int mode, double tau_mix, double tau_skip, double dmax = 0.0);
+    int precision() const { return precision_; }
+    // Quartets of the last build by task flag (0 FP64, 1 FP32, 2 skipped).
+    void flag_counts(long long *quartets3, long long *prim3);
 
     // G = J - K/2 for a symmetric density (device pointers, nao*nao, row-major).
     void fock_device(const double *D_dev, double *G_dev, hipStream_t st);
@@ -112,6 +122,17 @@ private:
     int *d_ao_off_ = nullptr;
     JKTask *d_tasks_ = nullptr;
     JKGroup *d_groups_ = nullptr;
+    int *d_task_group_ = nullptr;        // group of each task
+    unsigned char *d_flags_ = nullptr;   // per task, per build
+    unsigned int *d_gmask_w_ = nullptr;  // per group, per build
+    unsigned char *d_gmask_ = nullptr;
+    double *d_dsh_ = nullptr;            // shell-block max |D|, per build
+    int precision_ = 0;
+    double tau_mix_ = 0.0, tau_skip_ = 0.0, dmax_ = 0.0;
+    double *d_dmax_ = nullptr;
+    bool classified_ = false;            // flags valid for the last build
+    std::vector<JKTask> h_tasks_;        // host copy (counts by flag)
+    int ngroups_ = 0;
     int *d_counters_ = nullptr;  // one work counter per class launch
     int max_lds_ = 0;
     bool no_group_ = false;      // LIBRINT_NO_GROUP: flat per-task kernels only

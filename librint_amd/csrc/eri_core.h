@@ -171,67 +171,146 @@ __device__ __forceinline__ double rcp_pos(double x)
     return fma(y, e, y);
 }
 
-// LDS stride (doubles) of one table segment: the (DEG+1)*NV coefficients
-// padded so that the stride is 16 bytes past a multiple of 128 bytes.  Lanes
-// of a wavefront read the same coefficient row of different segments; with
-// this stride consecutive segments start in different 16-byte bank groups
-// instead of all aliasing onto one group.
-__host__ __device__ constexpr int rys_seg_stride(int ncoef)
+// Reciprocal square root in the working precision.
+__device__ __forceinline__ double rsqrt_t(double x) { return rsqrt_pos(x); }
+__device__ __forceinline__ float rsqrt_t(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// Rys table block of one root count staged in LDS, in the working precision
+// T: double (rys_table.inc, degree 11) or float (rys_table_f32.inc, degree 5).
+// rt = [seg][c][2N] (v then w), or for WONLY (all-s quartets, weights only)
+// [seg][c]; segments STRIDE elements apart, STRIDE padded so that it is 16
+// bytes past a multiple of 128 bytes: lanes of a wavefront read the same
+// coefficient row of different segments, and this keeps consecutive segments
+// in different LDS bank groups.
+template <typename T, int N, bool WONLY>
+struct RysTab {
+    static constexpr int NV = WONLY ? 1 : 2 * N;
+    static constexpr int DEG = sizeof(T) == 8 ? RYS_DEG : RYS32_DEG;
+    static constexpr int NCOEF = (DEG + 1) * NV;
+    static constexpr int EPB = 128 / (int)sizeof(T);  // elements per 128 bytes
+    static constexpr int PAD16 = 16 / (int)sizeof(T); // elements per 16 bytes
+    static constexpr int STRIDE = NCOEF + ((PAD16 - NCOEF % EPB) % EPB + EPB) % EPB;
+    static constexpr int LEN = RYS_NSEG[N] * STRIDE;
+};
+
+// copy the block of root count N into LDS (all threads of the block take part)
+template <typename T, int N, bool WONLY>
+__device__ __forceinline__ void rys_stage(T *rt)
 {
-    return ncoef + ((2 - ncoef % 16) + 16) % 16;
+    using R = RysTab<T, N, WONLY>;
+    for (int i = threadIdx.x; i < R::LEN; i += blockDim.x) {
+        const int seg = i / R::STRIDE, c = i - seg * R::STRIDE;
+        const int g = seg * R::NCOEF + c;  // index in the unpadded (weights-only) block
+        T v = (T)0;
+        if (c < R::NCOEF) {
+            if constexpr (sizeof(T) == 8)
+                v = R::NV == 2 * N ? (T)RYS_CHEB[RYS_OFF[N] + g] : (T)RYS_CHEB[RYS_OFF[N] + 2 * g + 1];
+            else
+                v = R::NV == 2 * N ? (T)RYS32_TAB[RYS32_OFF[N] + g] : (T)RYS32_TAB[RYS32_OFF[N] + 2 * g + 1];
+        }
+        rt[i] = v;
+    }
 }
 
-// Rys nodes/weights from a table block of one root count staged in LDS:
-// rt = [seg][c][2N] (v then w), or for WONLY = [seg][c] weights only, with
-// segments rys_seg_stride apart.
-template <int N, bool WONLY>
-__device__ __forceinline__ void rys_nodes_lds(double x, const double *rt, double *v, double *w)
+template <typename T>
+__device__ __forceinline__ T estrin_t(const T *a, int st, T s, T s2, T s4, T s8)
 {
-    constexpr int NV = WONLY ? 1 : 2 * N;
-    constexpr int STRIDE = rys_seg_stride((RYS_DEG + 1) * NV);
-    if (x >= (double)RYS_NSEG[N]) {
-        const double rs = rsqrt_pos(x);
-        const double rx = rs * rs;
+    if constexpr (sizeof(T) == 8) {
+        return rys_estrin11(a, st, s, s2, s4, s8);
+    } else {
+        const T q0 = fma(a[st], s, a[0]);
+        const T q1 = fma(a[3 * st], s, a[2 * st]);
+        const T q2 = fma(a[5 * st], s, a[4 * st]);
+        return fma(q2, s4, fma(q1, s2, q0));
+    }
+}
+
+template <typename T, int N, bool WONLY>
+__device__ __forceinline__ void rys_nodes_lds(T x, const T *rt, T *v, T *w)
+{
+    using R = RysTab<T, N, WONLY>;
+    if (x >= (T)RYS_NSEG[N]) {
+        const T rs = rsqrt_t(x);
+        const T rx = rs * rs;
 #pragma unroll
         for (int r = 0; r < N; r++) {
-            v[r] = WONLY ? 0.0 : RYS_HERM_NODE2[RYS_HOFF[N] + r] * rx;
-            w[r] = RYS_HERM_W[RYS_HOFF[N] + r] * rs;
+            v[r] = WONLY ? (T)0 : (T)RYS_HERM_NODE2[RYS_HOFF[N] + r] * rx;
+            w[r] = (T)RYS_HERM_W[RYS_HOFF[N] + r] * rs;
         }
         return;
     }
     const int k = (int)x;
-    const double s = 2.0 * (x - (double)k) - 1.0;
-    const double s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
-    const double *c = rt + k * STRIDE;
+    const T s = (T)2 * (x - (T)k) - (T)1;
+    const T s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
+    const T *c = rt + k * R::STRIDE;
     if constexpr (WONLY) {
-        w[0] = rys_estrin11(c, 1, s, s2, s4, s8);
-        v[0] = 0.0;
+        w[0] = estrin_t<T>(c, 1, s, s2, s4, s8);
+        v[0] = (T)0;
     } else {
 #pragma unroll
         for (int j = 0; j < N; j++) {
-            v[j] = rys_estrin11(c + j, NV, s, s2, s4, s8);
-            w[j] = rys_estrin11(c + N + j, NV, s, s2, s4, s8);
+            v[j] = estrin_t<T>(c + j, R::NV, s, s2, s4, s8);
+            w[j] = estrin_t<T>(c + N + j, R::NV, s, s2, s4, s8);
         }
     }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename T = double>
 struct RysLds {
     using C = Class<LA, LB, LC, LD>;
     static constexpr int N = C::NROOTS;
     static constexpr bool WONLY = (C::NMAX + C::MMAX) == 0;
-    static constexpr int NV = WONLY ? 1 : 2 * N;
-    static constexpr int NCOEF = (RYS_DEG + 1) * NV;       // per segment
-    static constexpr int STRIDE = rys_seg_stride(NCOEF);    // padded, in LDS
-    static constexpr int LEN = RYS_NSEG[N] * STRIDE;
+    using Tab = RysTab<T, N, WONLY>;
+    static constexpr int LEN = Tab::LEN;
 };
+
+// One primitive quartet, all roots, in working precision T with the Rys
+// table block in LDS; accumulated into eri.  rbd = Rb - Rd, rab = Rb - Ra,
+// rcd = Rd - Rc.
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, const PrimPair &q, const T rbd[3],
+                                                 const T rab[3], const T rcd[3], const T *rt)
+{
+    using C = Class<LA, LB, LC, LD>;
+    const T aij = (T)p.e, akl = (T)q.e;
+    const T rsq = rsqrt_t(aij + akl);
+    const T rs = rsq * rsq;
+    const T a0 = aij * akl * rs;
+    const T pq[3] = {(T)p.pbx - (T)q.pbx + rbd[0], (T)p.pby - (T)q.pby + rbd[1], (T)p.pbz - (T)q.pbz + rbd[2]};
+    const T rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
+    const T fac1 = (T)p.K * (T)q.K * rsq;
+    T v[C::NROOTS], w[C::NROOTS];
+    rys_nodes_lds<T, C::NROOTS, RysLds<LA, LB, LC, LD, T>::WONLY>(a0 * rr, rt, v, w);
+#pragma unroll
+    for (int r = 0; r < C::NROOTS; r++) {
+        const T tr = v[r] * rs;
+        const T b00 = (T)0.5 * tr;
+        const T b10 = (T)p.h * ((T)1 - akl * tr);
+        const T b01 = (T)q.h * ((T)1 - aij * tr);
+        const T ta = akl * tr;
+        const T tc = aij * tr;
+        T gx[C::GS], gy[C::GS], gz[C::GS];
+        gx[0] = (T)1;
+        gy[0] = (T)1;
+        gz[0] = w[r] * fac1;
+        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gx, (T)p.pbx - ta * pq[0], (T)q.pbx + tc * pq[0], b10, b01, b00);
+        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gy, (T)p.pby - ta * pq[1], (T)q.pby + tc * pq[1], b10, b01, b00);
+        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gz, (T)p.pbz - ta * pq[2], (T)q.pbz + tc * pq[2], b10, b01, b00);
+        if constexpr (LA > 0 || LC > 0) {
+            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gx, rab[0], rcd[0]);
+            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gy, rab[1], rcd[1]);
+            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gz, rab[2], rcd[2]);
+        }
+        gout_acc<LA, LB, LC, LD>(eri, gx, gy, gz);
+    }
+}
 
 // One primitive quartet, all roots, accumulated into eri (double precision).
 // rbd = Rb - Rd, rab = Rb - Ra, rcd = Rd - Rc.
-// LDS_TAB: Rys table block staged in LDS at rt (see RysLds), else global table.
-template <int LA, int LB, int LC, int LD, bool LDS_TAB>
+// One primitive quartet with the Rys table read from global memory (double).
+template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p, const PrimPair &q, const double rbd[3],
-                                                  const double rab[3], const double rcd[3], const double *rt)
+                                                  const double rab[3], const double rcd[3])
 {
     using C = Class<LA, LB, LC, LD>;
     const double aij = p.e, akl = q.e;
@@ -242,10 +321,7 @@ __device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p
     const double rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
     const double fac1 = p.K * q.K * rsq;
     double v[C::NROOTS], w[C::NROOTS];
-    if constexpr (LDS_TAB)
-        rys_nodes_lds<C::NROOTS, RysLds<LA, LB, LC, LD>::WONLY>(a0 * rr, rt, v, w);
-    else
-        rys_nodes<C::NROOTS>(a0 * rr, v, w);
+    rys_nodes<C::NROOTS>(a0 * rr, v, w);
 #pragma unroll
     for (int r = 0; r < C::NROOTS; r++) {
         // v = t^2;  b00 = t^2/(2(aij+akl)), b10 = (1 - akl t^2/(aij+akl))/(2 aij), ...

@@ -26,7 +26,10 @@ polynomials of its segment with contiguous loads):
     RYS_CHEB[off(n) + (seg*(DEG+1) + c)*2n + j],  j < n: v_j, j >= n: w_{j-n}
 with c the power of s.
 
-Run:  python librint_amd/csrc/gen_rys_table.py   (writes rys_table.inc next to it)
+Run:  python librint_amd/csrc/gen_rys_table.py         (writes rys_table.inc)
+      python librint_amd/csrc/gen_rys_table.py --f32   (writes rys_table_f32.inc:
+      degree 5, single precision, for the FP32 recurrence path; same segments
+      and cutoffs, arrays named RYS32_*)
 The output is committed; the generator needs mpmath and is not run at build time.
 """
 import os
@@ -35,8 +38,9 @@ import multiprocessing as mpr
 
 import mpmath as mp
 
+F32 = "--f32" in sys.argv
 NMAX = 9          # roots up to an all-g quartet (l <= 4)
-DEG = 11          # Chebyshev degree per segment
+DEG = 5 if F32 else 11  # polynomial degree per segment
 SEGLEN = 1.0      # segment length in x
 DPS = 50
 
@@ -135,7 +139,17 @@ def _fma(a, b, c):
 
 
 def estrin(a, s):
-    """Estrin evaluation of sum_k a[k] s^k (degree 11), the device's order."""
+    """Estrin evaluation of sum_k a[k] s^k (degree 11 or 5), the device's order."""
+    if len(a) == 6:
+        # single precision: round every step to float32 as the device does
+        import numpy as np
+        f = np.float32
+        a = [f(x) for x in a]
+        s = f(s)
+        s2 = f(s * s)
+        s4 = f(s2 * s2)
+        q0, q1, q2 = f(a[1] * s + a[0]), f(a[3] * s + a[2]), f(a[5] * s + a[4])
+        return float(f(f(q2 * s4) + f(q1 * s2 + q0)))
     s2 = s * s
     s4 = s2 * s2
     s8 = s4 * s4
@@ -185,6 +199,15 @@ def main():
                 flat.extend(row)
             worst[n][0] = max(worst[n][0], w[0])
             worst[n][1] = max(worst[n][1], w[1])
+    if F32:
+        write_f32(xcut, nseg, off, herm, flat, worst)
+    else:
+        write_f64(xcut, nseg, off, herm, flat, worst)
+    for n in range(1, NMAX + 1):
+        print("n=%d xcut=%d nseg=%d  max|dv|=%.2e max|dw|=%.2e" % (n, xcut[n], nseg[n], worst[n][0], worst[n][1]))
+
+
+def write_f64(xcut, nseg, off, herm, flat, worst):
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rys_table.inc")
     with open(path, "w") as f:
         f.write("// Generated by gen_rys_table.py -- piecewise polynomial Rys nodes/weights\n")
@@ -214,8 +237,25 @@ def main():
         for i in range(0, len(flat), 4):
             f.write("  " + ", ".join("%.17g" % v for v in flat[i:i + 4]) + ",\n")
         f.write("};\n")
-    for n in range(1, NMAX + 1):
-        print("n=%d xcut=%d nseg=%d  max|dv|=%.2e max|dw|=%.2e" % (n, xcut[n], nseg[n], worst[n][0], worst[n][1]))
+
+
+def write_f32(xcut, nseg, off, herm, flat, worst):
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rys_table_f32.inc")
+    with open(path, "w") as f:
+        f.write("// Generated by gen_rys_table.py --f32 -- single-precision piecewise polynomial\n")
+        f.write("// Rys nodes/weights (degree %d in the local variable, monomial basis) for the\n" % DEG)
+        f.write("// FP32 recurrence path.  Max abs error of float Estrin evaluation vs %d-digit\n" % DPS)
+        f.write("// Golub-Welsch, per n (v, w):\n")
+        for n in range(1, NMAX + 1):
+            f.write("//   n=%d  %.2e  %.2e\n" % (n, worst[n][0], worst[n][1]))
+        f.write("// Same segments and large-x cutoffs as rys_table.inc; include after defining\n")
+        f.write("// RYS_STORAGE.\n#define RYS32_DEG %d\n" % DEG)
+        f.write("RYS_STORAGE int RYS32_OFF[%d] = {0, %s};\n" % (NMAX + 1, ", ".join(str(off[n]) for n in range(1, NMAX + 1))))
+        f.write("#define RYS32_LEN %d\n" % len(flat))
+        f.write("RYS_STORAGE float RYS32_TAB[RYS32_LEN] = {\n")
+        for i in range(0, len(flat), 6):
+            f.write("  " + ", ".join("%.9ef" % v for v in flat[i:i + 6]) + ",\n")
+        f.write("};\n")
 
 
 if __name__ == "__main__":

@@ -46,17 +46,28 @@ struct ListTask {
 };
 
 bool class_supported(int la, int lb, int lc, int ld);
+// Task flags: 0 FP64, 1 FP32 recurrences, 2 skipped; a launch processes the
+// tasks whose flag bit is set in `accept` (flags == nullptr: all tasks).
 // counter: one int per launch, zero before the launch (launch_zero_counters)
-hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, int *counter, const DevBasis &B,
-                     const double *D, double *M, int nao, hipStream_t st);
+hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tasks, int ntask, int *counter,
+                     const unsigned char *flags, int accept, const DevBasis &B, const double *D, double *M, int nao,
+                     hipStream_t st);
 hipError_t launch_zero_counters(int *p, int n, hipStream_t st);
+bool class_has_fp32(int la, int lb, int lc, int ld);
 // grouped form (exchange rows of the bra in LDS); jk_group_lds gives its LDS
 // bytes per workgroup, (size_t)-1 for classes without a grouped kernel
-size_t jk_group_lds(int la, int lb, int lc, int ld, int nao);
-hipError_t launch_jk_group(int la, int lb, int lc, int ld, const JKGroup *groups, int ngroup, const JKTask *tasks,
-                           int *counter, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st);
+size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao);
+hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
+                           const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
+                           int accept, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st);
+// per-build task classification (shell-block max |D|, then flags and group masks)
+hipError_t launch_classify(const JKTask *tasks, int ntask, const int *task_group, int ngroup, const DevBasis &B,
+                           const double *D, const int *ao_off, int nsh, int nao, double *Dsh, double tau_mix,
+                           double tau_skip, unsigned char *flags, unsigned int *gmask_w, unsigned char *gmask,
+                           hipStream_t st);
 hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
                        double *out, int mode, int nao, hipStream_t st);
+hipError_t launch_absmax(const double *D, size_t n, double *out, hipStream_t st);
 hipError_t launch_symmetrize(const double *M, double *G, int nao, hipStream_t st);
 
 } // namespace librint

@@ -69,7 +69,7 @@ __device__ __forceinline__ void contracted_fp64(double *eri, const DevBasis &B, 
         const PrimPair p = B.pp[bra.off + ib];
         for (int ik = 0; ik < ket.npp; ik++) {
             const PrimPair q = B.pp[ket.off + ik];
-            prim_quartet_fp64<LA, LB, LC, LD, false>(eri, p, q, rbd, bra.rab, ket.rab, nullptr);
+            prim_quartet_fp64<LA, LB, LC, LD>(eri, p, q, rbd, bra.rab, ket.rab);
         }
     }
 }
@@ -82,39 +82,56 @@ __device__ __forceinline__ void contracted_fp64(double *eri, const DevBasis &B, 
 //     M[a][d] -= s/2 D[b][c] v      M[b][d] -= s/2 D[a][c] v
 // which sums the J and K contributions of all 8 index images.
 //
-// Persistent form: 256-thread workgroups stage the Rys table block of the
-// class's root count in LDS once, then each wavefront pulls tasks from a
-// per-launch counter.  In a task the bra pair is wavefront-uniform (its
-// primitive records become scalar loads) and each lane keeps its ket
-// primitive record in registers while the bra primitives stream past.
-template <int LA, int LB, int LC, int LD>
+// Task flags (per build, from classify_kernel): 0 FP64, 1 FP32 recurrences,
+// 2 skipped (density-weighted bound below the screening threshold).  A
+// launch processes the tasks whose flag bit is set in `accept`; flags ==
+// nullptr accepts every task.
+//
+// Working precision T: the primitive loop, Rys evaluation and the
+// contracted block run in T; the J/K contraction and every accumulation
+// into M are FP64.
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void contract_block(const JKTask &t, const PairView &bra, const PairView &ket, int ketp,
+                                               const DevBasis &B, T *eri, const T *rt)
+{
+    using C = Class<LA, LB, LC, LD>;
+#pragma unroll
+    for (int f = 0; f < C::NF; f++) eri[f] = (T)0;
+    const T rbd[3] = {(T)(bra.rb[0] - ket.rb[0]), (T)(bra.rb[1] - ket.rb[1]), (T)(bra.rb[2] - ket.rb[2])};
+    const T rab[3] = {(T)bra.rab[0], (T)bra.rab[1], (T)bra.rab[2]};
+    const T rcd[3] = {(T)ket.rab[0], (T)ket.rab[1], (T)ket.rab[2]};
+    // every ket of a task comes from one bin, so the primitive-pair count is
+    // the same in all lanes: make the loop bounds scalar for the compiler
+    const int knpp = __builtin_amdgcn_readfirstlane(ket.npp);
+    const int bnpp = __builtin_amdgcn_readfirstlane(bra.npp);
+    const int boff = __builtin_amdgcn_readfirstlane(bra.off);
+    for (int ik = 0; ik < knpp; ik++) {
+        const PrimPair q = B.pp[ket.off + ik];
+        for (int ib = 0; ib < bnpp; ib++) {
+            const PrimPair p = B.pp[boff + ib];
+            prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, rbd, rab, rcd, rt);
+        }
+    }
+}
+
+template <int LA, int LB, int LC, int LD, typename T>
 __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, const double *__restrict__ D,
-                                        double *__restrict__ M, int nao, int lane, const double *rt)
+                                        double *__restrict__ M, int nao, int lane, const T *rt)
 {
     using C = Class<LA, LB, LC, LD>;
     const bool active = lane < t.nket;
     const int ketp = t.ket0 + (active ? lane : 0);
     const PairView bra = load_pair(B, t.bra);
     const PairView ket = load_pair(B, ketp);
-
-    double eri[C::NF];
-#pragma unroll
-    for (int f = 0; f < C::NF; f++) eri[f] = 0.0;
-    const double rbd[3] = {bra.rb[0] - ket.rb[0], bra.rb[1] - ket.rb[1], bra.rb[2] - ket.rb[2]};
-    for (int ik = 0; ik < ket.npp; ik++) {
-        const PrimPair q = B.pp[ket.off + ik];
-        for (int ib = 0; ib < bra.npp; ib++) {
-            const PrimPair p = B.pp[bra.off + ib];
-            prim_quartet_fp64<LA, LB, LC, LD, true>(eri, p, q, rbd, bra.rab, ket.rab, rt);
-        }
-    }
+    T eri[C::NF];
+    contract_block<LA, LB, LC, LD, T>(t, bra, ket, ketp, B, eri, rt);
 
     double s = active ? 1.0 : 0.0;
     if (bra.sa == bra.sb) s *= 0.5;
     if (ket.sa == ket.sb) s *= 0.5;
     if (t.bra == ketp) s *= 0.5;
-
     const size_t n = (size_t)nao;
+#define ERI_(fa, fb, fc, fd) ((double)eri[(fa) + C::NFA * ((fb) + C::NFB * ((fc) + C::NFC * (fd)))])
     // bra-bra block: reduced over the wavefront, one atomic per element
 #pragma unroll
     for (int fa = 0; fa < C::NFA; fa++)
@@ -124,14 +141,11 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fc = 0; fc < C::NFC; fc++)
-                    acc += D[(ket.aoa + fc) * n + ket.aob + fd] *
-                           eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fc = 0; fc < C::NFC; fc++) acc += D[(ket.aoa + fc) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
             acc = wave_sum(2.0 * s * acc);
             if (lane == 0) atomicAdd(&M[(bra.aoa + fa) * n + bra.aob + fb], acc);
         }
     if (!active) return;
-    // ket-ket block
 #pragma unroll
     for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
@@ -140,13 +154,10 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
 #pragma unroll
             for (int fb = 0; fb < C::NFB; fb++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++)
-                    acc += D[(bra.aoa + fa) * n + bra.aob + fb] *
-                           eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + bra.aob + fb] * ERI_(fa, fb, fc, fd);
             atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], 2.0 * s * acc);
         }
     const double h = -0.5 * s;
-    // exchange: (a,c) and (b,c)
 #pragma unroll
     for (int fc = 0; fc < C::NFC; fc++) {
 #pragma unroll
@@ -155,9 +166,7 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fb = 0; fb < C::NFB; fb++)
-                    acc += D[(bra.aob + fb) * n + ket.aob + fd] *
-                           eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fb = 0; fb < C::NFB; fb++) acc += D[(bra.aob + fb) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
             atomicAdd(&M[(bra.aoa + fa) * n + ket.aoa + fc], h * acc);
         }
 #pragma unroll
@@ -166,13 +175,10 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++)
-                    acc += D[(bra.aoa + fa) * n + ket.aob + fd] *
-                           eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
             atomicAdd(&M[(bra.aob + fb) * n + ket.aoa + fc], h * acc);
         }
     }
-    // exchange: (a,d) and (b,d)
 #pragma unroll
     for (int fd = 0; fd < C::NFD; fd++) {
 #pragma unroll
@@ -181,9 +187,7 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
 #pragma unroll
             for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
-                for (int fb = 0; fb < C::NFB; fb++)
-                    acc += D[(bra.aob + fb) * n + ket.aoa + fc] *
-                           eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fb = 0; fb < C::NFB; fb++) acc += D[(bra.aob + fb) * n + ket.aoa + fc] * ERI_(fa, fb, fc, fd);
             atomicAdd(&M[(bra.aoa + fa) * n + ket.aob + fd], h * acc);
         }
 #pragma unroll
@@ -192,28 +196,24 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
 #pragma unroll
             for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++)
-                    acc += D[(bra.aoa + fa) * n + ket.aoa + fc] *
-                           eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + ket.aoa + fc] * ERI_(fa, fb, fc, fd);
             atomicAdd(&M[(bra.aob + fb) * n + ket.aob + fd], h * acc);
         }
     }
+#undef ERI_
 }
 
-template <int LA, int LB, int LC, int LD>
+// Persistent form: 256-thread workgroups stage the Rys table block of the
+// class's root count in LDS once, then each wavefront pulls tasks from a
+// per-launch counter.
+template <int LA, int LB, int LC, int LD, typename T>
 __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ tasks, int ntask, int *__restrict__ counter,
-                                                 DevBasis B, const double *__restrict__ D, double *__restrict__ M, int nao)
+                                                 const unsigned char *__restrict__ flags, int accept, DevBasis B,
+                                                 const double *__restrict__ D, double *__restrict__ M, int nao)
 {
-    using R = RysLds<LA, LB, LC, LD>;
-    __shared__ double rt[R::LEN];
-    {
-        const double *src = RYS_CHEB + RYS_OFF[R::N];
-        for (int i = threadIdx.x; i < R::LEN; i += blockDim.x) {
-            const int seg = i / R::STRIDE, c = i - seg * R::STRIDE;
-            const int g = seg * R::NCOEF + c;  // index in the unpadded (WONLY: weights-only) block
-            rt[i] = (c >= R::NCOEF) ? 0.0 : (R::WONLY ? src[2 * g + 1] : src[g]);
-        }
-    }
+    using R = RysLds<LA, LB, LC, LD, T>;
+    __shared__ T rt[R::LEN];
+    rys_stage<T, R::N, R::WONLY>(rt);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     for (;;) {
@@ -221,7 +221,8 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
         if (lane == 0) t = atomicAdd(counter, 1);
         t = __builtin_amdgcn_readfirstlane(t);
         if (t >= ntask) break;
-        jk_task<LA, LB, LC, LD>(tasks[t], B, D, M, nao, lane, rt);
+        if (flags && !((accept >> flags[t]) & 1)) continue;
+        jk_task<LA, LB, LC, LD, T>(tasks[t], B, D, M, nao, lane, rt);
     }
 }
 
@@ -233,34 +234,24 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
 // flushed once per group with coalesced global atomics; the bra-bra Coulomb
 // block is reduced over the whole group.  Only the ket-ket Coulomb block
 // (M[c][d], one atomic per element per quartet) goes straight to memory.
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename T>
 __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &bra, const DevBasis &B,
                                              const double *__restrict__ D, double *__restrict__ M, int nao, int lane,
-                                             const double *rt, double *Krow, double *mab)
+                                             const T *rt, double *Krow, double *mab)
 {
     using C = Class<LA, LB, LC, LD>;
     const bool active = lane < t.nket;
     const int ketp = t.ket0 + (active ? lane : 0);
     const PairView ket = load_pair(B, ketp);
-
-    double eri[C::NF];
-#pragma unroll
-    for (int f = 0; f < C::NF; f++) eri[f] = 0.0;
-    const double rbd[3] = {bra.rb[0] - ket.rb[0], bra.rb[1] - ket.rb[1], bra.rb[2] - ket.rb[2]};
-    for (int ik = 0; ik < ket.npp; ik++) {
-        const PrimPair q = B.pp[ket.off + ik];
-        for (int ib = 0; ib < bra.npp; ib++) {
-            const PrimPair p = B.pp[bra.off + ib];
-            prim_quartet_fp64<LA, LB, LC, LD, true>(eri, p, q, rbd, bra.rab, ket.rab, rt);
-        }
-    }
+    T eri[C::NF];
+    contract_block<LA, LB, LC, LD, T>(t, bra, ket, ketp, B, eri, rt);
     if (!active) return;
     double s = 1.0;
     if (bra.sa == bra.sb) s *= 0.5;
     if (ket.sa == ket.sb) s *= 0.5;
     if (t.bra == ketp) s *= 0.5;
     const size_t n = (size_t)nao;
-    // bra-bra Coulomb: per-lane partial sums, reduced at the end of the group
+#define ERI_(fa, fb, fc, fd) ((double)eri[(fa) + C::NFA * ((fb) + C::NFB * ((fc) + C::NFC * (fd)))])
 #pragma unroll
     for (int fa = 0; fa < C::NFA; fa++)
 #pragma unroll
@@ -269,11 +260,9 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fc = 0; fc < C::NFC; fc++)
-                    acc += D[(ket.aoa + fc) * n + ket.aob + fd] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fc = 0; fc < C::NFC; fc++) acc += D[(ket.aoa + fc) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
             mab[fa * C::NFB + fb] += 2.0 * s * acc;
         }
-    // ket-ket Coulomb
 #pragma unroll
     for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
@@ -282,8 +271,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fb = 0; fb < C::NFB; fb++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++)
-                    acc += D[(bra.aoa + fa) * n + bra.aob + fb] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + bra.aob + fb] * ERI_(fa, fb, fc, fd);
             atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], 2.0 * s * acc);
         }
     const double h = -0.5 * s;
@@ -296,8 +284,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fb = 0; fb < C::NFB; fb++)
-                    acc += D[(bra.aob + fb) * n + ket.aob + fd] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fb = 0; fb < C::NFB; fb++) acc += D[(bra.aob + fb) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
             atomicAdd(&Ka[fa * n + ket.aoa + fc], h * acc);
         }
 #pragma unroll
@@ -306,8 +293,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++)
-                    acc += D[(bra.aoa + fa) * n + ket.aob + fd] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
             atomicAdd(&Kb[fb * n + ket.aoa + fc], h * acc);
         }
     }
@@ -319,8 +305,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
-                for (int fb = 0; fb < C::NFB; fb++)
-                    acc += D[(bra.aob + fb) * n + ket.aoa + fc] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fb = 0; fb < C::NFB; fb++) acc += D[(bra.aob + fb) * n + ket.aoa + fc] * ERI_(fa, fb, fc, fd);
             atomicAdd(&Ka[fa * n + ket.aob + fd], h * acc);
         }
 #pragma unroll
@@ -329,51 +314,63 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++)
-                    acc += D[(bra.aoa + fa) * n + ket.aoa + fc] * eri[fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd))];
+                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + ket.aoa + fc] * ERI_(fa, fb, fc, fd);
             atomicAdd(&Kb[fb * n + ket.aob + fd], h * acc);
         }
     }
+#undef ERI_
 }
 
-template <int LA, int LB, int LC, int LD>
+// LDS footprint of the Rys table block in doubles, rounded up to 16 bytes
+template <int LA, int LB, int LC, int LD, typename T>
+__host__ __device__ constexpr int RysLdsDoubles()
+{
+    return (int)((RysLds<LA, LB, LC, LD, T>::LEN * sizeof(T) + 15) / 16 * 2);
+}
+
+// gmask: per group, bit f set when the group holds a task with flag f
+template <int LA, int LB, int LC, int LD, typename T>
 __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
                                                        const JKTask *__restrict__ tasks, int *__restrict__ counter,
-                                                       DevBasis B, const double *__restrict__ D,
-                                                       double *__restrict__ M, int nao)
+                                                       const unsigned char *__restrict__ flags,
+                                                       const unsigned char *__restrict__ gmask, int accept, DevBasis B,
+                                                       const double *__restrict__ D, double *__restrict__ M, int nao)
 {
     using C = Class<LA, LB, LC, LD>;
-    using R = RysLds<LA, LB, LC, LD>;
-    extern __shared__ double lds[];
-    double *rt = lds;                       // Rys table block
-    double *mab_l = lds + R::LEN;           // bra-bra block
-    double *Krow = mab_l + C::NFA * C::NFB; // (NFA + NFB) rows of nao
+    using R = RysLds<LA, LB, LC, LD, T>;
+    extern __shared__ __align__(16) double lds[];
+    T *rt = (T *)lds;                                       // Rys table block (16-byte aligned)
+    double *mab_l = lds + RysLdsDoubles<LA, LB, LC, LD, T>(); // bra-bra block
+    double *Krow = mab_l + C::NFA * C::NFB;                 // (NFA + NFB) rows of nao
     __shared__ int g_s;
-    {
-        const double *src = RYS_CHEB + RYS_OFF[R::N];
-        for (int i = threadIdx.x; i < R::LEN; i += blockDim.x) {
-            const int seg = i / R::STRIDE, c = i - seg * R::STRIDE;
-            const int g = seg * R::NCOEF + c;
-            rt[i] = (c >= R::NCOEF) ? 0.0 : (R::WONLY ? src[2 * g + 1] : src[g]);
-        }
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    rys_stage<T, R::N, R::WONLY>(rt);
+    const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nrow = (C::NFA + C::NFB) * nao;
     const size_t n = (size_t)nao;
     for (;;) {
-        if (threadIdx.x == 0) g_s = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) {
+            int g;
+            do {
+                g = atomicAdd(counter, 1);
+            } while (g < ngroup && gmask && !(gmask[g] & accept));
+            g_s = g;
+        }
         for (int i = threadIdx.x; i < nrow; i += blockDim.x) Krow[i] = 0.0;
         if (threadIdx.x < C::NFA * C::NFB) mab_l[threadIdx.x] = 0.0;
         __syncthreads();
-        const int g = g_s;
+        // the value is the same in every lane; readfirstlane tells the compiler
+        // so, which turns the bra loads below into scalar loads
+        const int g = __builtin_amdgcn_readfirstlane(g_s);
         if (g >= ngroup) break;
         const JKGroup G = groups[g];
         const PairView bra = load_pair(B, G.bra);
         double mab[C::NFA * C::NFB];
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) mab[e] = 0.0;
-        for (int t = G.first + wave; t < G.first + G.count; t += 4)
-            jk_task_rows<LA, LB, LC, LD>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab);
+        for (int t = G.first + wave; t < G.first + G.count; t += 4) {
+            if (flags && !((accept >> flags[t]) & 1)) continue;
+            jk_task_rows<LA, LB, LC, LD, T>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab);
+        }
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) {
             const double v = wave_sum(mab[e]);
@@ -394,34 +391,34 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
     }
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename T>
 static size_t group_lds_bytes(int nao)
 {
     using C = Class<LA, LB, LC, LD>;
-    using R = RysLds<LA, LB, LC, LD>;
-    return sizeof(double) * ((size_t)R::LEN + C::NFA * C::NFB + (size_t)(C::NFA + C::NFB) * nao);
+    return sizeof(double) * ((size_t)RysLdsDoubles<LA, LB, LC, LD, T>() + C::NFA * C::NFB + (size_t)(C::NFA + C::NFB) * nao);
 }
 
-template <int LA, int LB, int LC, int LD>
+template <int LA, int LB, int LC, int LD, typename T>
 static hipError_t launch_jk_group_t(const JKGroup *groups, int ngroup, const JKTask *tasks, int *counter,
+                                    const unsigned char *flags, const unsigned char *gmask, int accept,
                                     const DevBasis &B, const double *D, double *M, int nao, hipStream_t st)
 {
-    const size_t lds = group_lds_bytes<LA, LB, LC, LD>(nao);
+    const size_t lds = group_lds_bytes<LA, LB, LC, LD, T>(nao);
     static int grid = 0, grid_nao = -1;
     if (grid == 0 || grid_nao != nao) {
         int dev = 0, ncu = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_group_kernel<LA, LB, LC, LD>, 256, lds);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_group_kernel<LA, LB, LC, LD, T>, 256, lds);
         grid = std::max(1, ncu) * std::max(1, per);
         grid_nao = nao;
         if (getenv("LIBRINT_DEBUG"))
-            fprintf(stderr, "librint: jk_group<%d%d|%d%d> lds %zu blocks/cu %d grid %d\n", LA, LB, LC, LD, lds, per,
-                    grid);
+            fprintf(stderr, "librint: jk_group<%d%d|%d%d>%s lds %zu blocks/cu %d grid %d\n", LA, LB, LC, LD,
+                    sizeof(T) == 4 ? " fp32" : "", lds, per, grid);
     }
     const int g = std::min(grid, ngroup);
-    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks, counter, B,
-                       D, M, nao);
+    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, T>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks, counter,
+                       flags, gmask, accept, B, D, M, nao);
     return hipGetLastError();
 }
 
@@ -544,7 +541,8 @@ __device__ __forceinline__ void team_indices(int lane, int *pk)
 }
 
 template <int LA, int LB, int LC, int LD>
-__global__ __launch_bounds__(64) void jk_team_kernel(const JKTask *__restrict__ tasks, DevBasis B,
+__global__ __launch_bounds__(64) void jk_team_kernel(const JKTask *__restrict__ tasks,
+                                                     const unsigned char *__restrict__ flags, int accept, DevBasis B,
                                                      const double *__restrict__ D, double *__restrict__ M, int nao)
 {
     using C = Class<LA, LB, LC, LD>;
@@ -553,6 +551,7 @@ __global__ __launch_bounds__(64) void jk_team_kernel(const JKTask *__restrict__ 
     __shared__ double mab[S::NAB];
     __shared__ double mk[S::NKET];
     __shared__ double accs[S::NPL * 64];  // contracted outputs, read back one at a time for J/K
+    if (flags && !((accept >> flags[blockIdx.x]) & 1)) return;
     const JKTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x;
     const PairView bra = load_pair(B, t.bra);
@@ -737,6 +736,105 @@ __global__ __launch_bounds__(64) void eri_list_kernel(const ListTask *__restrict
     }
 }
 
+// Shell-pair block maxima of |D|: Dsh[i*nsh + j] = max |D[mu][nu]|, mu in
+// shell i, nu in shell j (virtual segmented shells).
+__global__ void dsh_kernel(const double *__restrict__ D, const int *__restrict__ ao_off, int nsh, int nao,
+                           double *__restrict__ Dsh)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (long long)nsh * nsh) return;
+    const int i = (int)(idx / nsh), j = (int)(idx % nsh);
+    const int i0 = ao_off[i], i1 = ao_off[i + 1], j0 = ao_off[j], j1 = ao_off[j + 1];
+    double m = 0.0;
+    for (int a = i0; a < i1; a++)
+        for (int b = j0; b < j1; b++) m = fmax(m, fabs(D[(size_t)a * nao + b]));
+    Dsh[idx] = m;
+}
+
+// One wavefront per task: the density-weighted Schwarz bound of each quartet
+// Q_ab Q_cd max(|D| over the six blocks the quartet reads) decides the task
+// flag from the largest bound of its kets: < tau_skip -> 2 (skip),
+// < tau_mix -> 1 (FP32 recurrences), else 0 (FP64).  gmask[group] collects
+// the flags present in each group.
+__global__ __launch_bounds__(256) void classify_kernel(const JKTask *__restrict__ tasks, int ntask,
+                                                       const int *__restrict__ task_group, DevBasis B,
+                                                       const double *__restrict__ Dsh, int nsh, double tau_mix,
+                                                       double tau_skip, unsigned char *__restrict__ flags,
+                                                       unsigned int *__restrict__ gmask)
+{
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (t >= ntask) return;
+    const JKTask T = tasks[t];
+    const int4 bi = B.pair_info[T.bra];
+    const double qb = B.pair_rb[T.bra].w;
+    double bound = 0.0;
+    if (lane < T.nket) {
+        const int4 ki = B.pair_info[T.ket0 + lane];
+        const double qk = B.pair_rb[T.ket0 + lane].w;
+        const size_t a = bi.x, b = bi.y, c = ki.x, d = ki.y, n = nsh;
+        double dm = fmax(fmax(Dsh[a * n + b], Dsh[c * n + d]), fmax(Dsh[a * n + c], Dsh[a * n + d]));
+        dm = fmax(dm, fmax(Dsh[b * n + c], Dsh[b * n + d]));
+        bound = qb * qk * dm;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) bound = fmax(bound, __shfl_xor(bound, off, 64));
+    if (lane == 0) {
+        const unsigned char f = bound < tau_skip ? 2 : (bound < tau_mix ? 1 : 0);
+        flags[t] = f;
+        atomicOr(&gmask[task_group[t]], 1u << f);
+    }
+}
+
+__global__ void zero_u32_kernel(unsigned int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// gmask words -> bytes for the group kernels
+__global__ void pack_gmask_kernel(const unsigned int *__restrict__ w, unsigned char *__restrict__ b, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (unsigned char)w[i];
+}
+
+hipError_t launch_classify(const JKTask *tasks, int ntask, const int *task_group, int ngroup, const DevBasis &B,
+                           const double *D, const int *ao_off, int nsh, int nao, double *Dsh, double tau_mix,
+                           double tau_skip, unsigned char *flags, unsigned int *gmask_w, unsigned char *gmask,
+                           hipStream_t st)
+{
+    const long long nn = (long long)nsh * nsh;
+    hipLaunchKernelGGL(dsh_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, D, ao_off, nsh, nao, Dsh);
+    if (ngroup > 0)
+        hipLaunchKernelGGL(zero_u32_kernel, dim3((ngroup + 255) / 256), dim3(256), 0, st, gmask_w, ngroup);
+    if (ntask > 0)
+        hipLaunchKernelGGL(classify_kernel, dim3((ntask + 3) / 4), dim3(256), 0, st, tasks, ntask, task_group, B, Dsh,
+                           nsh, tau_mix, tau_skip, flags, gmask_w);
+    if (ngroup > 0)
+        hipLaunchKernelGGL(pack_gmask_kernel, dim3((ngroup + 255) / 256), dim3(256), 0, st, gmask_w, gmask, ngroup);
+    return hipGetLastError();
+}
+
+// max |D| over n doubles (non-negative doubles order like their bit patterns)
+__global__ void absmax_kernel(const double *__restrict__ D, size_t n, unsigned long long *__restrict__ out)
+{
+    double m = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs(D[i]));
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+hipError_t launch_absmax(const double *D, size_t n, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, (unsigned int *)out, 2);
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(absmax_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, st, D, n, (unsigned long long *)out);
+    return hipGetLastError();
+}
+
 __global__ void symmetrize_kernel(const double *__restrict__ M, double *__restrict__ G, int nao)
 {
     const size_t n = (size_t)nao;
@@ -771,24 +869,25 @@ bool class_supported(int la, int lb, int lc, int ld)
     return false;
 }
 
-template <int LA, int LB, int LC, int LD>
-static hipError_t launch_jk_persistent(const JKTask *tasks, int ntask, int *counter, const DevBasis &B, const double *D,
-                                       double *M, int nao, hipStream_t st)
+template <int LA, int LB, int LC, int LD, typename T>
+static hipError_t launch_jk_persistent(const JKTask *tasks, int ntask, int *counter, const unsigned char *flags,
+                                       int accept, const DevBasis &B, const double *D, double *M, int nao,
+                                       hipStream_t st)
 {
     static int grid = 0;
     if (grid == 0) {
         int dev = 0, ncu = 0, per = 0;
         (void)hipGetDevice(&dev);
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_kernel<LA, LB, LC, LD>, 256, 0);
-        int mult = 1;
-        if (const char *e = getenv("LIBRINT_JK_GRID_MULT")) mult = std::max(1, atoi(e));
-        grid = std::max(1, ncu) * std::max(1, per) * mult;
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_kernel<LA, LB, LC, LD, T>, 256, 0);
+        grid = std::max(1, ncu) * std::max(1, per);
         if (getenv("LIBRINT_DEBUG"))
-            fprintf(stderr, "librint: jk<%d%d|%d%d> ncu %d blocks/cu %d grid %d\n", LA, LB, LC, LD, ncu, per, grid);
+            fprintf(stderr, "librint: jk<%d%d|%d%d>%s blocks/cu %d grid %d\n", LA, LB, LC, LD,
+                    sizeof(T) == 4 ? " fp32" : "", per, grid);
     }
     const int g = std::min(grid, (ntask + 3) / 4);
-    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD>), dim3(g), dim3(256), 0, st, tasks, ntask, counter, B, D, M, nao);
+    hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, T>), dim3(g), dim3(256), 0, st, tasks, ntask, counter, flags, accept, B,
+                       D, M, nao);
     return hipGetLastError();
 }
 
@@ -805,17 +904,31 @@ hipError_t launch_zero_counters(int *p, int n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int ntask, int *counter, const DevBasis &B,
-                     const double *D, double *M, int nao, hipStream_t st)
+bool class_has_fp32(int la, int lb, int lc, int ld)
+{
+#define X(a, b, c, d)                                                                                                  \
+    if (la == a && lb == b && lc == c && ld == d) return true;
+    LR_REG_CLASSES(X)
+#undef X
+    return false;
+}
+
+hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tasks, int ntask, int *counter,
+                     const unsigned char *flags, int accept, const DevBasis &B, const double *D, double *M, int nao,
+                     hipStream_t st)
 {
     if (ntask == 0) return hipSuccess;
 #define X(a, b, c, d)                                                                                                  \
-    if (la == a && lb == b && lc == c && ld == d) return launch_jk_persistent<a, b, c, d>(tasks, ntask, counter, B, D, M, nao, st);
+    if (la == a && lb == b && lc == c && ld == d)                                                                      \
+        return fp32 ? launch_jk_persistent<a, b, c, d, float>(tasks, ntask, counter, flags, accept, B, D, M, nao, st)  \
+                    : launch_jk_persistent<a, b, c, d, double>(tasks, ntask, counter, flags, accept, B, D, M, nao, st);
     LR_REG_CLASSES(X)
 #undef X
+    if (fp32) return hipErrorInvalidValue;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d) {                                                                    \
-        hipLaunchKernelGGL((jk_team_kernel<a, b, c, d>), dim3(ntask), dim3(64), 0, st, tasks, B, D, M, nao);          \
+        hipLaunchKernelGGL((jk_team_kernel<a, b, c, d>), dim3(ntask), dim3(64), 0, st, tasks, flags, accept, B, D, M,   \
+                           nao);                                                                                       \
         return hipGetLastError();                                                                                      \
     }
     LR_TEAM_CLASSES(X)
@@ -823,22 +936,27 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, const JKTask *tasks, int nt
     return hipErrorInvalidValue;
 }
 
-size_t jk_group_lds(int la, int lb, int lc, int ld, int nao)
+size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao)
 {
 #define X(a, b, c, d)                                                                                                  \
-    if (la == a && lb == b && lc == c && ld == d) return group_lds_bytes<a, b, c, d>(nao);
+    if (la == a && lb == b && lc == c && ld == d)                                                                      \
+        return fp32 ? group_lds_bytes<a, b, c, d, float>(nao) : group_lds_bytes<a, b, c, d, double>(nao);
     LR_REG_CLASSES(X)
 #undef X
     return (size_t)-1;
 }
 
-hipError_t launch_jk_group(int la, int lb, int lc, int ld, const JKGroup *groups, int ngroup, const JKTask *tasks,
-                           int *counter, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st)
+hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
+                           const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
+                           int accept, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st)
 {
     if (ngroup == 0) return hipSuccess;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
-        return launch_jk_group_t<a, b, c, d>(groups, ngroup, tasks, counter, B, D, M, nao, st);
+        return fp32 ? launch_jk_group_t<a, b, c, d, float>(groups, ngroup, tasks, counter, flags, gmask, accept, B, D, \
+                                                           M, nao, st)                                                 \
+                    : launch_jk_group_t<a, b, c, d, double>(groups, ngroup, tasks, counter, flags, gmask, accept, B,   \
+                                                            D, M, nao, st);
     LR_REG_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;

@@ -29,6 +29,7 @@ namespace librint {
 #define RYS_STORAGE static const
 #endif
 #include "rys_table.inc"
+#include "rys_table_f32.inc"
 #undef RYS_STORAGE
 
 // Estrin evaluation of a degree-11 polynomial sum_c a[c*stride] s^c

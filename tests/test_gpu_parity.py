@@ -225,3 +225,55 @@ def test_tensor_d_shells(gpu):
     R = scf.int2e(m)
     Rr = refcpu.int2e(m)
     assert np.abs(R - Rr).max() < 1e-12
+
+
+def _core_density(m):
+    from librint_amd import scf
+    S = scf.int1e(m, "ovlp")
+    H = scf.int1e(m, "kin") + scf.int1e(m, "nuc")
+    s, U = np.linalg.eigh(S)
+    X = U / np.sqrt(s)
+    e, C = np.linalg.eigh(X.T @ H @ X)
+    C = X @ C
+    nocc = m.nelectron // 2
+    return 2.0 * C[:, :nocc] @ C[:, :nocc].T
+
+
+def test_mixed_precision_fock(gpu):
+    """FP32 recurrences below tau_mix: error bounded, energy error << 1 uEh."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("mixed precision C16H34")
+    m = mol.c16h34_631g()
+    ctx = FockContext(m, schwarz_tol=1e-12)
+    P = _core_density(m)
+    G64 = ctx.fock(P)
+    ctx.set_precision(1, 0.0)          # tau_mix 0: every task stays FP64
+    assert np.abs(ctx.fock(P) - G64).max() < 1e-12
+    for tau in (1e-8, 1e-6, 1e-5):
+        ctx.set_precision(1, tau)
+        Gm = ctx.fock(P)
+        fc = ctx.flag_counts()
+        assert fc["quartets"][1] > 0
+        dE = 0.5 * np.sum(P * (Gm - G64))
+        _say("tau_mix", tau, "fp32 quartets", fc["quartets"], "dE", dE, "max dG", np.abs(Gm - G64).max())
+        assert abs(dE) < 1e-6
+        assert np.abs(Gm - G64).max() < 1e-5
+    ctx.set_precision(0, 0.0, 1e-11)   # density screening only
+    Gs = ctx.fock(P)
+    assert ctx.flag_counts()["quartets"][2] > 0
+    assert abs(0.5 * np.sum(P * (Gs - G64))) < 1e-7
+
+
+def test_mixed_precision_d_shells(gpu):
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("mixed precision (H2O)4 even-tempered")
+    m = mol.water_cluster(4)
+    ctx = FockContext(m, schwarz_tol=1e-12)
+    P = _core_density(m)
+    G64 = ctx.fock(P)
+    ctx.set_precision(1, 1e-6)
+    Gm = ctx.fock(P)
+    assert ctx.flag_counts()["quartets"][1] > 0
+    assert abs(0.5 * np.sum(P * (Gm - G64))) < 1e-6
