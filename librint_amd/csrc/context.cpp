@@ -435,6 +435,7 @@ void Context::build_tasks()
     classified_ = false;
     dev_free(d_counters_);
     check(hipMalloc(&d_counters_, sizeof(int) * 2 * std::max<size_t>(launches_.size(), 1)), "hipMalloc");
+    check(hipMemset(d_counters_, 0, sizeof(int) * 2 * std::max<size_t>(launches_.size(), 1)), "hipMemset");
 }
 
 void Context::set_precision(int mode, double tau_mix, double tau_skip, double dmax)
@@ -467,16 +468,9 @@ void Context::flag_counts(long long *q3, long long *p3)
 void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
 {
     const size_t nn = (size_t)nao_ * nao_;
-    if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
-    check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
-    const DevBasis B = devb();
-    check(launch_zero_counters(d_counters_, 2 * (int)launches_.size(), st), "zero counters");
-    if (max_lds_ == 0) {
-        int dev = 0;
-        check(hipGetDevice(&dev), "hipGetDevice");
-        check(hipDeviceGetAttribute(&max_lds_, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
-        if (getenv("LIBRINT_DEBUG")) fprintf(stderr, "librint: max LDS per block %d\n", max_lds_);
-    }
+    // Mixed precision without a given dmax: take max|D| of this density and
+    // rebuild the quartet lists once.  This must come before anything below
+    // uses the task, group or counter buffers, which the rebuild reallocates.
     if (precision_ == 1 && tau_mix_ > 0.0 && dmax_ <= 0.0) {
         // split threshold from this density: max|D|, then rebuild the lists once
         if (!d_dmax_) check(hipMalloc(&d_dmax_, sizeof(double)), "hipMalloc");
@@ -486,6 +480,17 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         check(hipStreamSynchronize(st), "hipStreamSynchronize");
         dmax_ = dm > 0.0 ? dm : 1.0;
         build_tasks();
+    }
+
+    if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
+    check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
+    const DevBasis B = devb();
+    check(launch_zero_counters(d_counters_, 2 * (int)launches_.size(), st), "zero counters");
+    if (max_lds_ == 0) {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&max_lds_, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+        if (getenv("LIBRINT_DEBUG")) fprintf(stderr, "librint: max LDS per block %d\n", max_lds_);
     }
     const bool classify = tau_skip_ > 0.0;
     const int nsh = (int)shells_.size();

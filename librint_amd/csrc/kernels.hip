@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
         int t = 0;
         if (lane == 0) t = atomicAdd(counter, 1);
         t = __builtin_amdgcn_readfirstlane(t);
-        if (t >= ntask) break;
+        if (t >= ntask || t < 0) break;
         if (flags && !((accept >> flags[t]) & 1)) continue;
         jk_task<LA, LB, LC, LD, T>(tasks[t], B, D, M, nao, lane, rt);
     }
@@ -352,7 +352,7 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
             int g;
             do {
                 g = atomicAdd(counter, 1);
-            } while (g < ngroup && gmask && !(gmask[g] & accept));
+            } while (g >= 0 && g < ngroup && gmask && !(gmask[g] & accept));
             g_s = g;
         }
         for (int i = threadIdx.x; i < nrow; i += blockDim.x) Krow[i] = 0.0;
@@ -361,7 +361,7 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
         // the value is the same in every lane; readfirstlane tells the compiler
         // so, which turns the bra loads below into scalar loads
         const int g = __builtin_amdgcn_readfirstlane(g_s);
-        if (g >= ngroup) break;
+        if (g >= ngroup || g < 0) break;
         const JKGroup G = groups[g];
         const PairView bra = load_pair(B, G.bra);
         double mab[C::NFA * C::NFB];
