@@ -71,6 +71,69 @@ def cpu_baseline(m, P, seconds, seed=0):
     return rate, float(nb) ** 4 / rate, total, t_used
 
 
+def bench_grad(args):
+    """RHF energy gradient w.r.t. every nuclear coordinate and Gaussian
+    exponent: converged SCF (untimed), then K timed gradient evaluations at
+    that density (adjoint Rys kernel on the GPU + host one-electron parts).
+    Optionally checks against FD of re-converged energies (5-point central
+    stencil, steps 1e-4 bohr and 1e-4*max(zeta, 1) in the exponents; the
+    3-point quotient at the same step is reported beside it)."""
+    from librint_amd import autograd
+    name = args.workload if args.workload != "c16h34_631g" else "w8_et"
+    m, _ = workload(name)
+    tau = 1e-14
+    E0, P0, it, ctx = autograd.rhf(m, conv=1e-10, schwarz_tol=tau)
+    for _ in range(args.warmup):
+        autograd.gradient(ctx, P0)
+    t0 = time.perf_counter()
+    kms = []
+    for _ in range(args.steps):
+        gn, de = autograd.gradient(ctx, P0)
+        kms.append(ctx.stats()["grad_kernel_ms"])
+    wall = (time.perf_counter() - t0) / args.steps
+    st = ctx.stats()
+    slots = autograd.exponent_slots(m)
+    line = {
+        "metric": "RHF analytic gradient (nuclear + exponents) s/eval",
+        "value": wall, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "seeded synthetic water cluster, even-tempered s,p,d basis",
+        "config": {"workload": name, "nao": m.nao(), "natm": len(m._atm), "exponents": len(slots),
+                   "schwarz": tau, "quartets": int(st["quartets"]), "scf_energy": E0, "scf_iterations": it},
+        "adjoint_kernel_ms": float(np.mean(kms)),
+        "grad_norm_nuclear": float(np.linalg.norm(gn)),
+    }
+    ctx.close()
+    if args.fd != "0":
+        env0 = m._env.copy()
+        params = [("R", a, d, int(m._atm[a][1]) + d) for a in range(len(m._atm)) for d in range(3)]
+        params += [("exp", -1, -1, int(k)) for k in slots]
+        if args.fd != "all":
+            rng = np.random.default_rng(5)
+            params = [params[i] for i in rng.choice(len(params), int(args.fd), replace=False)]
+        errs, errs3 = [], []
+        for kind, a, d, k in params:
+            h = 1e-4 if kind == "R" else 1e-4 * max(env0[k], 1.0)
+            es = {}
+            for sg in (2, 1, -1, -2):
+                m._env[k] = env0[k] + sg * h
+                E, _, _, c = autograd.rhf(m, conv=1e-10, schwarz_tol=tau, P0=P0)
+                c.close()
+                es[sg] = E
+            m._env[:] = env0
+            fd3 = (es[1] - es[-1]) / (2 * h)
+            fd = (8.0 * (es[1] - es[-1]) - (es[2] - es[-2])) / (12.0 * h)
+            an = gn[a, d] if kind == "R" else de[k]
+            errs.append(abs(fd - an))
+            errs3.append(abs(fd3 - an))
+            print("fd %s %d %d env[%d] fd5 %.10f fd3 %.10f analytic %.10f diff5 %.2e diff3 %.2e"
+                  % (kind, a, d, k, fd, fd3, an, abs(fd - an), abs(fd3 - an)), file=sys.stderr, flush=True)
+        line["fd_check"] = {"params": len(params), "stencil": "5-point, h = 1e-4 bohr / 1e-4*max(zeta,1)",
+                            "max_abs_diff": float(max(errs)), "mean_abs_diff": float(np.mean(errs)),
+                            "max_abs_diff_3point": float(max(errs3))}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +145,10 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--tau-mix", type=float, default=1e-6)
     ap.add_argument("--tau-skip", type=float, default=0.0)
+    ap.add_argument("--grad", action="store_true",
+                    help="time the analytic RHF gradient (adjoint ERI kernel) instead of the Fock build")
+    ap.add_argument("--fd", default="0",
+                    help="with --grad: check 'all' or N seeded parameters against re-converged central FD")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the Fock build is sharded by quartet bins "
@@ -91,6 +158,8 @@ def main():
     import torch
     from librint_amd.context import FockContext
 
+    if args.grad:
+        return bench_grad(args)
     m, tau = workload(args.workload)
     P = core_guess_density(m)
     ctx = FockContext(m, schwarz_tol=tau)

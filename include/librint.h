@@ -52,9 +52,20 @@ double energy_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, siz
 double scf_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, size_t nelec, int imax,
              double conv);
 
-/* Basis-parameter gradients of the reference (src/p2c.rs:130-144, 239-394).
- * Not provided by this library yet: each returns NULL and says so on stderr,
- * so that bindings written for the reference still load. */
+/* Basis-parameter gradients of the reference, each the derivative with
+ * respect to env[s1..s2) (s1, s2 from the split rule of reference
+ * src/utils.rs:152-175: every primitive exponent and contraction
+ * coefficient), P the density they are evaluated at; release with free_c.
+ * The two-electron part runs on the GPU (adjoint Rys kernel), the
+ * one-electron parts on the host.
+ *   dS_u          nao*nao*(s2-s1), [(a*nao+b)*L + k] = dS_ab/denv[s1+k]
+ *                 reference src/p2c.rs:130-144, src/dscf.rs:96-150
+ *   grad_c        dHcore + dR            reference src/p2c.rs:240-258, src/dscf.rs:547-557
+ *   dS_c          sum_ab (PFP)_ab dS_ab  reference src/p2c.rs:267-285, src/dscf.rs:354-368
+ *   dHcore_c      sum_ab P_ab d(T+V)_ab  reference src/p2c.rs:294-312, src/dscf.rs:320-336
+ *   dR_c          d/denv 1/2 tr(P G[P])  reference src/p2c.rs:321-339, src/dscf.rs:370-524
+ *   danalytical_c dHcore + dR - dS/2     reference src/p2c.rs:348-366, src/dscf.rs:526-537
+ *   denergy_c     same as danalytical_c  reference src/p2c.rs:376-394, src/dscf.rs:559-565 */
 double *dS_u(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l);
 double *grad_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
 double *dS_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
@@ -112,9 +123,10 @@ size_t librint_int2e_shells(librint_ctx *ctx, const int *quartets, int nq, doubl
  * (nbas*(nbas+1)/2 doubles). Returns 0 on success. */
 int librint_schwarz(librint_ctx *ctx, double *out);
 
-/* Counters of the current quartet plan, out[0..7]:
+/* Counters of the current quartet plan, out[0..8]:
  * quartets, primitive quartets, contracted integrals, algorithmic flops,
- * candidate quartets, kernel launches, device ms of the last build, Schwarz tol.
+ * candidate quartets, kernel launches, device ms of the last build, Schwarz tol,
+ * device ms of the last adjoint (gradient) kernel launches.
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 
@@ -129,6 +141,21 @@ int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);
  * 2 caller density in P_inout.  diis: 0/1.  On return P_inout (nao*nao,
  * may be NULL) holds the final density and *iters the cycle count.
  * Returns the energy, NaN on failure. */
+/* RHF energy gradient at a converged density P (host, nao*nao):
+ *   gnuc[3*natm] = dE/dR  (core H incl. the point-charge positions, the
+ *                  two-electron adjoint, -1/2 tr(PFP dS), nuclear repulsion)
+ *   denv[env_l]  = dE/d(env entry) for every primitive exponent and
+ *                  contraction coefficient (dHcore + dR - dS/2)
+ * parts (optional): [gH | gR | gS | gN] (3*natm each) then [dH | dR | dS]
+ * (env_l each).  Quartets are screened with the context's Schwarz threshold.
+ * No counterpart in the reference (its gradients are basis-parameter only). */
+int librint_ctx_gradient(librint_ctx *ctx, const double *P, double *gnuc, double *denv, double *parts);
+
+/* Nuclear-coordinate RHF gradient (3*natm, release with free_c) at a
+ * converged density; stateless form of librint_ctx_gradient without
+ * screening.  No counterpart in the reference. */
+double *grad_nuc_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
+
 double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess, int diis, double *P_inout,
                    int *iters);
 

@@ -70,6 +70,10 @@ library.librint_ctx_flag_counts.restype = ctypes.c_int
 library.librint_scf.argtypes = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 ctypes.c_int, _dp, _ip)
 library.librint_scf.restype = ctypes.c_double
+library.librint_ctx_gradient.argtypes = (ctypes.c_void_p, _dp, _dp, _dp, _dp)
+library.librint_ctx_gradient.restype = ctypes.c_int
+library.grad_nuc_c.argtypes = _mol + (_dp, _sz)
+library.grad_nuc_c.restype = _dp
 library.librint_last_error.argtypes = ()
 library.librint_last_error.restype = ctypes.c_char_p
 
@@ -80,5 +84,5 @@ EXPORTED = (
     "librint_ctx_destroy", "librint_ctx_nao", "librint_ctx_set_schwarz", "librint_fock",
     "librint_fock_device", "librint_int2e_shells", "librint_schwarz", "librint_ctx_stats",
     "librint_ctx_class_stats", "librint_ctx_set_precision", "librint_ctx_flag_counts",
-    "librint_scf", "librint_last_error",
+    "librint_scf", "librint_last_error", "librint_ctx_gradient", "grad_nuc_c",
 )

@@ -7,7 +7,7 @@ from ._bindings import library
 from . import utils
 
 STAT_KEYS = ("quartets", "prim_quartets", "integrals", "flops", "candidates", "launches",
-             "kernel_ms", "schwarz_tol")
+             "kernel_ms", "schwarz_tol", "grad_kernel_ms")
 
 
 class FockContext:
@@ -86,8 +86,8 @@ class FockContext:
         return out
 
     def stats(self):
-        out = np.zeros(8)
-        library.librint_ctx_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 8)
+        out = np.zeros(len(STAT_KEYS))
+        library.librint_ctx_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(STAT_KEYS))
         return dict(zip(STAT_KEYS, out.tolist()))
 
     def class_stats(self):

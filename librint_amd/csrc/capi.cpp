@@ -6,10 +6,12 @@
 #include <exception>
 #include <memory>
 #include <string>
+#include <vector>
 
 #include "../../include/librint.h"
 #include "context.h"
 #include "int1e.h"
+#include "linalg.h"
 #include "scf.h"
 
 using namespace librint;
@@ -152,17 +154,208 @@ double scf_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t
     }
 }
 
-#define LR_NOT_PROVIDED(name)                                                                                          \
-    set_err(#name ": basis-parameter gradients are not provided by this library yet");                                 \
-    return nullptr;
+}  // extern "C"
 
-double *dS_u(int *, size_t, int *, size_t, double *, size_t) { LR_NOT_PROVIDED(dS_u) }
-double *grad_c(int *, size_t, int *, size_t, double *, size_t, double *, size_t) { LR_NOT_PROVIDED(grad_c) }
-double *dS_c(int *, size_t, int *, size_t, double *, size_t, double *, size_t) { LR_NOT_PROVIDED(dS_c) }
-double *dHcore_c(int *, size_t, int *, size_t, double *, size_t, double *, size_t) { LR_NOT_PROVIDED(dHcore_c) }
-double *dR_c(int *, size_t, int *, size_t, double *, size_t, double *, size_t) { LR_NOT_PROVIDED(dR_c) }
-double *danalytical_c(int *, size_t, int *, size_t, double *, size_t, double *, size_t) { LR_NOT_PROVIDED(danalytical_c) }
-double *denergy_c(int *, size_t, int *, size_t, double *, size_t, double *, size_t) { LR_NOT_PROVIDED(denergy_c) }
+namespace {
+
+// env split of the differentiable parameters (reference src/utils.rs:152-175):
+// s1 = smallest exponent offset; s2 = coefficient offset + nprim of the
+// shell that last raised the running maximum (same update rule).
+void ref_split(const int *bas, size_t bas_l, size_t &s1, size_t &s2)
+{
+    long long mn = -1, mx = -1;
+    for (size_t b = 0; b + 8 <= bas_l; b += 8) {
+        const long long ngto = bas[b + 2], ex = bas[b + 5], cont = bas[b + 6];
+        if (mn == -1 || mn > ex) mn = ex;
+        if (mx == -1) mx = cont + ngto;
+        else if (mx < cont) mx = cont + ngto;
+    }
+    s1 = mn < 0 ? 0 : (size_t)mn;
+    s2 = mx < 0 ? 0 : (size_t)mx;
+}
+
+// Parts of the basis-parameter gradient of E = tr(P H) + 1/2 tr(P G[P]) at a
+// converged density (reference src/dscf.rs:320-368, 517-537), each over the
+// full env (entries outside the basis parameters stay zero), plus the
+// nuclear-coordinate parts of the same terms.
+struct GradParts {
+    std::vector<double> dH, dR, dS;     // env-length
+    std::vector<double> gH, gR, gS, gN; // 3*natm: core H, 2e, overlap (Pulay), nuclear repulsion
+};
+
+enum { GP_H = 1, GP_R = 2, GP_S = 4 };
+
+GradParts grad_parts(Context *ctx, const Int1eInput &one, const double *P, size_t env_l, int what)
+{
+    GradParts g;
+    const size_t na = one.charges.size();
+    const int n = one.nao;
+    const size_t nn = (size_t)n * n;
+    if (what & GP_H) {
+        g.dH.assign(env_l, 0.0);
+        g.gH.assign(3 * na, 0.0);
+        int1e_grad(one, 1, P, g.gH.data(), g.dH.data(), env_l);
+        int1e_grad(one, 2, P, g.gH.data(), g.dH.data(), env_l);
+    }
+    if (what & GP_R) {
+        g.dR.assign(env_l, 0.0);
+        g.gR.assign(3 * na, 0.0);
+        ctx->grad2e(P, g.gR.data(), g.dR.data(), env_l);
+    }
+    if (what & GP_S) {
+        // Q = P F P with F = T + V + G[P]
+        std::vector<double> F(nn), T(nn), G(nn), PF(nn), Q(nn);
+        int1e_matrix(one, 1, T.data());
+        int1e_matrix(one, 2, F.data());
+        ctx->fock_host(P, G.data());
+        for (size_t k = 0; k < nn; k++) F[k] += T[k] + G[k];
+        gemm_nn(n, P, F.data(), PF.data());
+        gemm_nn(n, PF.data(), P, Q.data());
+        // symmetrise against rounding so that the bra-derivative identity holds
+        for (int a = 0; a < n; a++)
+            for (int b = 0; b < a; b++) {
+                const double v = 0.5 * (Q[(size_t)a * n + b] + Q[(size_t)b * n + a]);
+                Q[(size_t)a * n + b] = Q[(size_t)b * n + a] = v;
+            }
+        g.dS.assign(env_l, 0.0);
+        g.gS.assign(3 * na, 0.0);
+        int1e_grad(one, 0, Q.data(), g.gS.data(), g.dS.data(), env_l);
+    }
+    g.gN.assign(3 * na, 0.0);
+    nuclear_repulsion_grad(one, g.gN.data());
+    return g;
+}
+
+// One reference-ABI gradient call: build the context (no quartet screening,
+// as the reference differentiates every canonical quartet), check P, combine
+// the parts with the given weights and return the [s1, s2) slice.
+double *grad_entry(const char *name, int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l,
+                   double *P, size_t P_l, int what, double wH, double wR, double wS)
+{
+    try {
+        if (!atm || !bas || !env || !P) {
+            set_err(std::string(name) + ": null argument");
+            return nullptr;
+        }
+        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        const size_t n = (size_t)one.nao;
+        if (P_l != n * n) {
+            set_err(std::string(name) + ": P has length " + std::to_string(P_l) + ", expected " + std::to_string(n * n));
+            return nullptr;
+        }
+        // the GPU context is needed for the two-electron parts only
+        std::unique_ptr<Context> ctx;
+        if (what & (GP_R | GP_S)) {
+            ctx.reset(new Context(atm, atm_l, bas, bas_l, env, env_l));
+            ctx->set_schwarz(0.0);
+        }
+        GradParts g = grad_parts(ctx.get(), one, P, env_l, what);
+        size_t s1, s2;
+        ref_split(bas, bas_l, s1, s2);
+        if (s2 < s1 || s2 > env_l) {
+            set_err(std::string(name) + ": basis parameters lie outside env");
+            return nullptr;
+        }
+        double *out = alloc_out(s2 - s1);
+        for (size_t k = s1; k < s2; k++) {
+            double v = 0.0;
+            if (what & GP_H) v += wH * g.dH[k];
+            if (what & GP_R) v += wR * g.dR[k];
+            if (what & GP_S) v += wS * g.dS[k];
+            out[k - s1] = v;
+        }
+        return out;
+    } catch (std::exception &e) {
+        set_err(std::string(name) + ": " + e.what());
+        return nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+double *dS_u(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l)
+{
+    try {
+        if (!atm || !bas || !env) {
+            set_err("dS_u: null argument");
+            return nullptr;
+        }
+        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        size_t s1, s2;
+        ref_split(bas, bas_l, s1, s2);
+        if (s2 < s1 || s2 > env_l) {
+            set_err("dS_u: basis parameters lie outside env");
+            return nullptr;
+        }
+        const size_t n = (size_t)one.nao, L = s2 - s1;
+        double *out = alloc_out(n * n * L);
+        overlap_param_derivs(one, s1, L, out);
+        return out;
+    } catch (std::exception &e) {
+        set_err(std::string("dS_u: ") + e.what());
+        return nullptr;
+    }
+}
+
+double *grad_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l)
+{
+    return grad_entry("grad_c", atm, atm_l, bas, bas_l, env, env_l, P, P_l, GP_H | GP_R, 1.0, 1.0, 0.0);
+}
+
+double *dS_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l)
+{
+    return grad_entry("dS_c", atm, atm_l, bas, bas_l, env, env_l, P, P_l, GP_S, 0.0, 0.0, 1.0);
+}
+
+double *dHcore_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l)
+{
+    return grad_entry("dHcore_c", atm, atm_l, bas, bas_l, env, env_l, P, P_l, GP_H, 1.0, 0.0, 0.0);
+}
+
+double *dR_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l)
+{
+    return grad_entry("dR_c", atm, atm_l, bas, bas_l, env, env_l, P, P_l, GP_R, 0.0, 1.0, 0.0);
+}
+
+double *danalytical_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P,
+                      size_t P_l)
+{
+    return grad_entry("danalytical_c", atm, atm_l, bas, bas_l, env, env_l, P, P_l, GP_H | GP_R | GP_S, 1.0, 1.0,
+                      -0.5);
+}
+
+double *denergy_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l)
+{
+    return grad_entry("denergy_c", atm, atm_l, bas, bas_l, env, env_l, P, P_l, GP_H | GP_R | GP_S, 1.0, 1.0, -0.5);
+}
+
+double *grad_nuc_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l)
+{
+    try {
+        if (!atm || !bas || !env || !P) {
+            set_err("grad_nuc_c: null argument");
+            return nullptr;
+        }
+        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        const size_t n = (size_t)one.nao;
+        if (P_l != n * n) {
+            set_err("grad_nuc_c: P has the wrong length");
+            return nullptr;
+        }
+        Context ctx(atm, atm_l, bas, bas_l, env, env_l);
+        ctx.set_schwarz(0.0);
+        GradParts g = grad_parts(&ctx, one, P, env_l, GP_H | GP_R | GP_S);
+        const size_t m = g.gN.size();
+        double *out = alloc_out(m);
+        for (size_t k = 0; k < m; k++) out[k] = g.gH[k] + g.gR[k] - 0.5 * g.gS[k] + g.gN[k];
+        return out;
+    } catch (std::exception &e) {
+        set_err(std::string("grad_nuc_c: ") + e.what());
+        return nullptr;
+    }
+}
 
 double *fock_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l,
                int coord, int precision, double schwarz_tol)
@@ -332,10 +525,11 @@ int librint_ctx_stats(const librint_ctx *ctx, double *out, int n)
 {
     if (!ctx || !out) return -1;
     const Stats &s = ctx->ctx->stats();
-    const double v[8] = {(double)s.quartets, (double)s.prim_quartets, (double)s.integrals, s.flops,
-                         (double)s.candidates, (double)s.launches, (double)s.last_kernel_ms, ctx->ctx->schwarz()};
+    const double v[9] = {(double)s.quartets, (double)s.prim_quartets, (double)s.integrals, s.flops,
+                         (double)s.candidates, (double)s.launches, (double)s.last_kernel_ms, ctx->ctx->schwarz(),
+                         (double)s.last_grad_ms};
     int k = 0;
-    for (; k < n && k < 8; k++) out[k] = v[k];
+    for (; k < n && k < 9; k++) out[k] = v[k];
     return k;
 }
 
@@ -352,6 +546,36 @@ int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n)
         for (int j = 0; j < 10; j++) out[10 * k + j] = v[j];
     }
     return (int)L.size();
+}
+
+int librint_ctx_gradient(librint_ctx *ctx, const double *P, double *gnuc, double *denv, double *parts)
+{
+    try {
+        if (!ctx || !P) {
+            set_err("null argument");
+            return -1;
+        }
+        Context &c = *ctx->ctx;
+        const size_t env_l = c.env().size();
+        GradParts g = grad_parts(&c, ctx->one, P, env_l, GP_H | GP_R | GP_S);
+        const size_t m = g.gN.size();
+        if (gnuc)
+            for (size_t k = 0; k < m; k++) gnuc[k] = g.gH[k] + g.gR[k] - 0.5 * g.gS[k] + g.gN[k];
+        if (denv)
+            for (size_t k = 0; k < env_l; k++) denv[k] = g.dH[k] + g.dR[k] - 0.5 * g.dS[k];
+        if (parts) {
+            // [gH | gR | gS | gN] (each 3*natm) then [dH | dR | dS] (each env_l)
+            double *o = parts;
+            for (const auto *v : {&g.gH, &g.gR, &g.gS, &g.gN, &g.dH, &g.dR, &g.dS}) {
+                std::memcpy(o, v->data(), sizeof(double) * v->size());
+                o += v->size();
+            }
+        }
+        return 0;
+    } catch (std::exception &e) {
+        set_err(e.what());
+        return -1;
+    }
 }
 
 double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess, int diis, double *P_inout,

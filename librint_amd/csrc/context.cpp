@@ -4,6 +4,8 @@
 // binning and quartet task lists.
 #include "context.h"
 
+#include "grad.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -115,6 +117,9 @@ Context::~Context()
     dev_free(d_M_);
     dev_free(d_D_);
     dev_free(d_G_);
+    dev_free(d_pp_idx_);
+    dev_free(d_shell_slot_);
+    dev_free(d_slot_coef_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
@@ -137,6 +142,7 @@ void Context::build_pairs()
     const int ns = (int)shells_.size();
     pairs_.clear();
     pp_.clear();
+    pp_idx_.clear();
     pair_of_ij_.assign((size_t)ns * (ns + 1) / 2, -1);
     for (int i = 0; i < ns; i++)
         for (int j = 0; j <= i; j++) {
@@ -178,8 +184,11 @@ void Context::build_pairs()
                     r.pbx = (ap * A.r[0] + aq * Bs.r[0]) / e - Bs.r[0];
                     r.pby = (ap * A.r[1] + aq * Bs.r[1]) / e - Bs.r[1];
                     r.pbz = (ap * A.r[2] + aq * Bs.r[2]) / e - Bs.r[2];
-                    r.pad0 = r.pad1 = 0.0;
+                    r.pad0 = ap;  // exponents, read by the adjoint kernel
+                    r.pad1 = aq;
                     pp_.push_back(r);
+                    pp_idx_.push_back(p);
+                    pp_idx_.push_back(q);
                 }
             P.npp = (int)pp_.size() - P.off;
             P.Q = 0.0;
@@ -691,6 +700,62 @@ size_t Context::shells_host(const int *quartets, int nq, double *out)
                     }
     }
     return (size_t)out_total;
+}
+
+void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
+{
+    // per-shell primitive slots: exponent and coefficient derivatives are
+    // accumulated per (shell, primitive) and folded onto env entries here
+    const int ns = (int)shells_.size();
+    if (!d_pp_idx_) {
+        std::vector<int> slot(ns);
+        std::vector<int2> info(ns);
+        std::vector<double> cf;
+        int nslot = 0;
+        for (int s = 0; s < ns; s++) {
+            slot[s] = nslot;
+            info[s] = make_int2(shells_[s].atom, nslot);
+            for (int ip = 0; ip < shells_[s].nprim; ip++) cf.push_back(env_[shells_[s].pcoef + ip]);
+            nslot += shells_[s].nprim;
+        }
+        nslot_ = nslot;
+        std::vector<int2> idx(pp_.size());
+        for (size_t k = 0; k < pp_.size(); k++) idx[k] = make_int2(pp_idx_[2 * k], pp_idx_[2 * k + 1]);
+        d_pp_idx_ = dev_upload(idx);
+        d_shell_slot_ = dev_upload(info);
+        d_slot_coef_ = dev_upload(cf);
+    }
+    const size_t nn = (size_t)nao_ * nao_;
+    const int na = natm();
+    if (!d_D_) check(hipMalloc(&d_D_, sizeof(double) * nn), "hipMalloc");
+    check(hipMemcpy(d_D_, P, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    double *dg = nullptr;
+    const size_t nout = 3 * (size_t)na + 2 * (size_t)nslot_;
+    check(hipMalloc(&dg, sizeof(double) * nout), "hipMalloc");
+    check(hipMemset(dg, 0, sizeof(double) * nout), "hipMemset");
+    const DevBasis B = devb();
+    check(hipEventRecord(ev0_, 0), "hipEventRecord");
+    for (const ClassLaunch &L : launches_)
+        for (int side = 0; side < 2; side++)
+            check(launch_grad(L.la, L.lb, L.lc, L.ld, side, d_tasks_ + L.first, L.count, B, d_pp_idx_, d_shell_slot_,
+                              d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, 0),
+                  "grad launch");
+    check(hipEventRecord(ev1_, 0), "hipEventRecord");
+    std::vector<double> h(nout);
+    check(hipMemcpy(h.data(), dg, sizeof(double) * nout, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    check(hipEventElapsedTime(&stats_.last_grad_ms, ev0_, ev1_), "hipEventElapsedTime");
+    dev_free(dg);
+    for (int k = 0; k < 3 * na; k++) gatom[k] = h[k];
+    std::fill(denv, denv + env_l, 0.0);
+    const double *dp = h.data() + 3 * na;
+    int slot = 0;
+    for (int s = 0; s < ns; s++) {
+        const Shell &S = shells_[s];
+        for (int ip = 0; ip < S.nprim; ip++, slot++) {
+            if ((size_t)(S.pexp + ip) < env_l) denv[S.pexp + ip] += dp[2 * slot];
+            if ((size_t)(S.pcoef + ip) < env_l) denv[S.pcoef + ip] += dp[2 * slot + 1];
+        }
+    }
 }
 
 std::vector<double> Context::schwarz_pairs() const

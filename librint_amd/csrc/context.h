@@ -29,6 +29,7 @@ struct Stats {
     long long candidates = 0;      // canonical shell quartets before screening
     int launches = 0;
     float last_kernel_ms = 0.f;    // device time of the last build (events)
+    float last_grad_ms = 0.f;      // device time of the last adjoint (grad2e) launches
 };
 
 struct ClassLaunch {
@@ -75,6 +76,10 @@ public:
     size_t shells_host(const int *quartets, int nq, double *out);
     // Schwarz bound per shell pair (i >= j), host copy, index i*(i+1)/2 + j.
     std::vector<double> schwarz_pairs() const;
+    // Adjoint of E2 = 1/2 tr(P G[P]) over the screened quartet lists (FP64):
+    // gatom[3*natm] = dE2/dR, denv[env_l] = dE2/d(env entry) for every
+    // primitive exponent and contraction coefficient (zero elsewhere).
+    void grad2e(const double *P, double *gatom, double *denv, size_t env_l);
 
     const Stats &stats() const { return stats_; }
     const std::vector<Shell> &shells() const { return shells_; }
@@ -106,6 +111,7 @@ private:
     };
     std::vector<PairH> pairs_;
     std::vector<PrimPair> pp_;
+    std::vector<int> pp_idx_;      // primitive indices (on sa, on sb) per record
     std::vector<int> pair_of_ij_;  // canonical (i,j) -> position in pairs_
     struct Bin {
         int cls, npp, first, count;
@@ -137,6 +143,10 @@ private:
     int max_lds_ = 0;
     bool no_group_ = false;      // LIBRINT_NO_GROUP: flat per-task kernels only
     double *d_M_ = nullptr, *d_D_ = nullptr, *d_G_ = nullptr;
+    int2 *d_pp_idx_ = nullptr;      // adjoint kernel: primitive indices per record
+    int2 *d_shell_slot_ = nullptr;  // {atom, first primitive slot} per shell
+    double *d_slot_coef_ = nullptr; // contraction coefficient per slot
+    int nslot_ = 0;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<hipEvent_t> evc_;  // one event after each class launch
     DevBasis devb() const;

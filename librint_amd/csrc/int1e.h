@@ -7,7 +7,7 @@
 namespace librint {
 
 struct Shell1 {
-    int l, nprim, pexp, pcoef, ao;
+    int l, nprim, pexp, pcoef, ao, atom;
     double r[3];
 };
 
@@ -23,5 +23,18 @@ Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t
 // typ 0 overlap, 1 kinetic, 2 nuclear attraction; R[mu*nao + nu]
 void int1e_matrix(const Int1eInput &in, int typ, double *R);
 double nuclear_repulsion(const Int1eInput &in);
+
+// Derivatives of E1 = sum_ab W_ab O_ab (W symmetric, nao*nao; O = S (typ 0),
+// T (1) or V (2)) with respect to every primitive exponent and contraction
+// coefficient (added into denv[env index], entries below env_l) and to the
+// nuclear coordinates (added into gatom[3*natm]; for V this includes the
+// derivative with respect to the point-charge positions).  Either output may
+// be null.
+void int1e_grad(const Int1eInput &in, int typ, const double *W, double *gatom, double *denv, size_t env_l);
+// d/dO_ab of every overlap element: dS[(a*nao + b)*L + (e - s1)] for env
+// entries s1 <= e < s1 + L.
+void overlap_param_derivs(const Int1eInput &in, size_t s1, size_t L, double *dS);
+// gradient of the point-charge repulsion (added into gatom[3*natm])
+void nuclear_repulsion_grad(const Int1eInput &in, double *gatom);
 
 }  // namespace librint
