@@ -25,6 +25,14 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
 HBM_PEAK = 8000.0         # GB/s
 
+# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs scaled by the same counters on a copy of known size
+# (profiles/jk_pmc_traffic_c16h34_631g.md); keyed by (workload, precision, kernel).
+TRAFFIC = {
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 0, 0, 0, double>"): (7945.7 * 2.0 + 186409.6) * 1024,
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, double>"): (10932.9 * 2.0 + 264143.8) * 1024,
+}
+
 
 def core_guess_density(m):
     from librint_amd import scf
@@ -147,6 +155,8 @@ def main():
     ap.add_argument("--tau-skip", type=float, default=0.0)
     ap.add_argument("--grad", action="store_true",
                     help="time the analytic RHF gradient (adjoint ERI kernel) instead of the Fock build")
+    ap.add_argument("--copy-mb", type=float, default=0.0,
+                    help="only copy this many MiB device to device --steps times (PMC calibration)")
     ap.add_argument("--fd", default="0",
                     help="with --grad: check 'all' or N seeded parameters against re-converged central FD")
     args = ap.parse_args()
@@ -160,6 +170,17 @@ def main():
 
     if args.grad:
         return bench_grad(args)
+    if args.copy_mb > 0:
+        # device-to-device copy of a known size: the reference point for the
+        # FETCH_SIZE / WRITE_SIZE counters of the roofline traffic figure
+        n = int(args.copy_mb * 2 ** 20) // 8
+        a = torch.arange(n, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(max(args.steps, 1)):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "copies": max(args.steps, 1)}))
+        return
     m, tau = workload(args.workload)
     P = core_guess_density(m)
     ctx = FockContext(m, schwarz_tol=tau)
@@ -181,10 +202,11 @@ def main():
     wall = time.perf_counter() - t0
     # device time of the J/K kernels (HIP events recorded by the library on
     # its stream around the class launches), one more timed build each
-    ev_ms = []
+    ev_ms, cs_runs = [], []
     for _ in range(min(args.steps, 10)):
         ctx.fock(P)
         ev_ms.append(ctx.stats()["kernel_ms"])
+        cs_runs.append(ctx.class_stats())
     st = ctx.stats()
     err = None
     if G64 is not None:
@@ -195,14 +217,22 @@ def main():
                "max_abs_dG": float(np.abs(Gm - G64).max()),
                "quartets_fp64_fp32_skipped": fc["quartets"],
                "prim_quartets_fp64_fp32_skipped": fc["prim_quartets"]}
+    # per-launch device times from the library's HIP events, averaged over
+    # the event-timed builds
+    cstats = cs_runs[-1]
+    for k, c in enumerate(cstats):
+        c["ms"] = float(np.mean([r[k]["ms"] for r in cs_runs]))
     classes = [{"class": "(%d%d|%d%d)" % (int(c["la"]), int(c["lb"]), int(c["lc"]), int(c["ld"])),
-                "quartets": int(c["quartets"]), "prim_quartets": int(c["prim_quartets"]),
+                "kernel": c["kernel"], "quartets": int(c["quartets"]), "prim_quartets": int(c["prim_quartets"]),
                 "lane_util": c["quartets"] / max(c["lanes"], 1), "ms": c["ms"],
-                "tflops": c["flops"] / max(c["ms"], 1e-9) / 1e9} for c in ctx.class_stats()]
+                "tflops": c["flops"] / max(c["ms"], 1e-9) / 1e9} for c in cstats]
     ms_per_step = wall / args.steps * 1e3
     quartets = st["quartets"]
     jk_ms = float(np.mean(ev_ms))
-    achieved = st["flops"] / (jk_ms * 1e-3) / 1e12
+    # dominant kernel: the class launch with the largest device time
+    dom = max(cstats, key=lambda c: c["ms"])
+    dom_achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    achieved_all = st["flops"] / (jk_ms * 1e-3) / 1e12
     line = {
         "metric": "RHF Fock-build s/iter + ERI quartets/s (FP64 fused J/K)",
         "value": quartets / (ms_per_step * 1e-3),
@@ -221,10 +251,12 @@ def main():
                    "precision": "fp64", "quartets": int(quartets), "prim_quartets": int(st["prim_quartets"]),
                    "integrals": int(st["integrals"]), "candidates": int(st["candidates"]),
                    "launches": int(st["launches"])},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "jk_kernel (all angular classes of one build)",
-                     "kernel_ms": jk_ms, "flops_per_launch": st["flops"]},
+        "roofline": {"bound": "mfma", "achieved": dom_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": dom_achieved / FP64_PEAK_TFLOPS,
+                     "traffic": TRAFFIC.get((args.workload, args.precision, dom["kernel"])),
+                     "kernel": dom["kernel"], "kernel_ms": dom["ms"], "flops_per_launch": dom["flops"],
+                     "all_classes": {"achieved": achieved_all, "frac": achieved_all / FP64_PEAK_TFLOPS,
+                                     "ms": jk_ms, "flops": st["flops"]}},
         "classes": classes,
     }
     if err is not None:

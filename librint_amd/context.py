@@ -93,10 +93,22 @@ class FockContext:
     def class_stats(self):
         """Per angular-class launch: counts, flops and the last build's device ms."""
         n = library.librint_ctx_class_stats(self.h, None, 0)
-        out = np.zeros(10 * max(n, 1))
+        out = np.zeros(12 * max(n, 1))
         library.librint_ctx_class_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
-        keys = ("la", "lb", "lc", "ld", "quartets", "prim_quartets", "flops", "lanes", "ms", "tasks")
-        return [dict(zip(keys, out[10 * k:10 * k + 10].tolist())) for k in range(n)]
+        keys = ("la", "lb", "lc", "ld", "quartets", "prim_quartets", "flops", "lanes", "ms", "tasks", "fp32",
+                "variant")
+        rows = [dict(zip(keys, out[12 * k:12 * k + 12].tolist())) for k in range(n)]
+        names = {0: "jk_kernel", 1: "jk_group_kernel", 2: "jk_team_kernel"}
+        for r in rows:
+            v = int(r["variant"])
+            ls = "%d, %d, %d, %d" % (r["la"], r["lb"], r["lc"], r["ld"])
+            if v == 2:
+                r["kernel"] = "jk_team_kernel<%s>" % ls
+            elif v in names:
+                r["kernel"] = "%s<%s, %s>" % (names[v], ls, "float" if r["fp32"] else "double")
+            else:
+                r["kernel"] = None
+        return rows
 
     def scf(self, imax=200, conv=1e-6, guess=0, diis=False, P0=None):
         P = np.zeros((self.nao, self.nao)) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()

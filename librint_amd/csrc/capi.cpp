@@ -541,9 +541,10 @@ int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n)
     int k = 0;
     for (; k < (int)L.size() && k < n; k++) {
         const ClassLaunch &c = L[k];
-        const double v[10] = {(double)c.la, (double)c.lb, (double)c.lc, (double)c.ld, (double)c.quartets,
-                              (double)c.prim_quartets, c.flops, (double)c.lanes, (double)c.ms, (double)c.count};
-        for (int j = 0; j < 10; j++) out[10 * k + j] = v[j];
+        const double v[12] = {(double)c.la,        (double)c.lb,    (double)c.lc,      (double)c.ld,
+                              (double)c.quartets,  (double)c.prim_quartets, c.flops,  (double)c.lanes,
+                              (double)c.ms,        (double)c.count, (double)c.fp32,    (double)c.variant};
+        for (int j = 0; j < 12; j++) out[12 * k + j] = v[j];
     }
     return (int)L.size();
 }

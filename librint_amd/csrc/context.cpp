@@ -514,10 +514,12 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     const unsigned char *gmask = classify ? d_gmask_ : nullptr;
     check(hipEventRecord(ev0_, st), "hipEventRecord");
     for (size_t k = 0; k < launches_.size(); k++) {
-        const ClassLaunch &L = launches_[k];
+        ClassLaunch &L = launches_[k];
         int *ctr = d_counters_ + 2 * k;
         const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_);
-        if (!no_group_ && lds <= (size_t)max_lds_)
+        const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
+        L.variant = grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2);
+        if (grouped)
             check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
                                   gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, st),
                   "jk group launch");
