@@ -90,6 +90,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     }
     nao_ = ao;
     no_group_ = getenv("LIBRINT_NO_GROUP") != nullptr;
+    no_jslab_ = getenv("LIBRINT_NO_JSLAB") != nullptr;
     check(hipEventCreate(&ev0_), "hipEventCreate");
     check(hipEventCreate(&ev1_), "hipEventCreate");
     build_pairs();
@@ -117,6 +118,9 @@ Context::~Context()
     dev_free(d_M_);
     dev_free(d_D_);
     dev_free(d_G_);
+    dev_free(d_J_);
+    dev_free(d_items_);
+    dev_free(d_item_tasks_);
     dev_free(d_pp_idx_);
     dev_free(d_shell_slot_);
     dev_free(d_slot_coef_);
@@ -314,12 +318,13 @@ void Context::build_tasks()
     stats_ = Stats();
     const long long np = (long long)pairs_.size();
     stats_.candidates = np * (np + 1) / 2;
-    // A group = all tasks of one bra pair for one ket class (capped so that a
-    // heavy bra is spread over a few workgroups).
-    int GROUP_MAX_TASKS = 8;
+    // A group = consecutive tasks of one bra pair for one ket class; its size
+    // is set per launch below (LIBRINT_GROUP_TASKS overrides).
+    int GROUP_MAX_TASKS = 0;
     if (const char *e = getenv("LIBRINT_GROUP_TASKS")) GROUP_MAX_TASKS = std::max(1, atoi(e));
     struct Acc {
-        std::vector<JKTask> tasks;
+        std::vector<JKTask> tasks;   // contiguous per bra pair
+        std::vector<double> cost;    // primitive quartets per task
         std::vector<std::pair<double, JKGroup>> groups;
         ClassLaunch stat;
     };
@@ -332,7 +337,6 @@ void Context::build_tasks()
         for (int i = 0; i < bx.count; i++) {
             const int p = bx.first + i;
             const double Qp = pairs_[p].Q;
-            std::map<std::pair<int, int>, int> open;  // (ket class, fp32) -> index of the open group
             for (int Y = 0; Y <= X; Y++) {
                 const Bin &by = bins_[Y];
                 int lc, ld;
@@ -360,7 +364,9 @@ void Context::build_tasks()
                         if (Qp * pairs_[by.first + mid].Q * dmax_ >= tau_mix_) l2 = mid + 1;
                         else h2 = mid;
                     }
-                    n64 = l2;
+                    // round the split up to a whole 64-ket window (those
+                    // kets stay FP64) so every task starts on a window
+                    n64 = std::min(n, (l2 + 63) / 64 * 64);
                 }
                 for (int part = 0; part < 2; part++) {
                     const int k_begin = part ? n64 : 0, k_end = part ? n : n64;
@@ -377,17 +383,9 @@ void Context::build_tasks()
                     A.stat.flops += fl;
                     A.stat.lanes += 64LL * ((nn + 63) / 64);
                     for (int k0 = k_begin; k0 < k_end; k0 += 64) {
-                        const std::pair<int, int> key{by.cls, part};
-                        auto it = open.find(key);
-                        if (it == open.end() || A.groups[it->second].second.count >= GROUP_MAX_TASKS) {
-                            A.groups.push_back({0.0, JKGroup{p, (int)A.tasks.size(), 0, 0}});
-                            open[key] = (int)A.groups.size() - 1;
-                            it = open.find(key);
-                        }
                         const int nk = std::min(64, k_end - k0);
                         A.tasks.push_back(JKTask{p, by.first + k0, nk, 0});
-                        A.groups[it->second].second.count++;
-                        A.groups[it->second].first += (double)nk * bx.npp * by.npp;
+                        A.cost.push_back((double)nk * bx.npp * by.npp);
                     }
                 }
             }
@@ -398,6 +396,30 @@ void Context::build_tasks()
     std::vector<int> task_group;
     for (auto &kv : acc) {
         Acc &A = kv.second;
+        // Group size: a group's exchange rows ((NFA+NFB) x nao in LDS) are
+        // flushed with global atomics once per group, so a group should hold
+        // about (NFA+NFB)*nao/64 tasks for the flush to stay small next to its
+        // quartets; at most 256, and small enough to leave >= 4096 groups per
+        // launch for load balance (never below 8).
+        int cap = GROUP_MAX_TASKS;
+        if (cap <= 0) {
+            int la, lb;
+            cls_ls(std::get<0>(kv.first), la, lb);
+            const int rows = ncart(la) + ncart(lb);
+            cap = std::min(256, std::max(8, (rows * nao_ + 63) / 64));
+            cap = std::max(8, std::min(cap, (int)(A.tasks.size() / 4096)));
+        }
+        for (size_t t = 0; t < A.tasks.size();) {
+            const int bra = A.tasks[t].bra;
+            JKGroup G{bra, (int)t, 0, 0};
+            double c = 0.0;
+            while (t < A.tasks.size() && A.tasks[t].bra == bra && G.count < cap) {
+                c += A.cost[t];
+                G.count++;
+                t++;
+            }
+            A.groups.push_back({c, G});
+        }
         // heavy groups first for load balance
         std::stable_sort(A.groups.begin(), A.groups.end(),
                          [](const std::pair<double, JKGroup> &x, const std::pair<double, JKGroup> &y) {
@@ -424,6 +446,67 @@ void Context::build_tasks()
         if (L.count) launches_.push_back(L);
     }
     stats_.launches = (int)launches_.size();
+
+    // Coulomb slabs: every launch of a register class stores its ket-side
+    // Coulomb blocks per task; items = (launch, 64-ket window) with the list
+    // of the launch's tasks on that window.
+    std::vector<JRedItem> items;
+    std::vector<int> item_tasks;
+    const size_t JBUF_MAX = (size_t)1 << 28;  // doubles (2 GiB)
+    size_t total = 0, largest = 0;
+    max_nfcd_ = 1;
+    for (ClassLaunch &L : launches_) {
+        L.jbase = -1;
+        L.ifirst = L.icount = 0;
+        if (no_jslab_ || !class_has_fp32(L.la, L.lb, L.lc, L.ld)) continue;
+        const size_t sz = (size_t)L.count * 64 * ncart(L.lc) * ncart(L.ld);
+        if (sz > JBUF_MAX) continue;
+        L.jbase = (long long)total;
+        total += sz;
+        largest = std::max(largest, sz);
+    }
+    jslab_whole_ = total <= JBUF_MAX;
+    for (ClassLaunch &L : launches_) {
+        if (L.jbase < 0) continue;
+        if (!jslab_whole_) L.jbase = 0;
+        std::map<int, std::vector<int>> win;
+        for (int t = L.first; t < L.first + L.count; t++) win[tasks[t].ket0].push_back(t);
+        L.ifirst = (int)items.size();
+        // a window's task list is split into items of at most JRED_TASKS
+        // tasks, so that the reduction has enough workgroups
+        const int JRED_TASKS = 256;
+        for (auto &kv : win) {
+            const int cnt = (int)kv.second.size();
+            item_tasks.insert(item_tasks.end(), kv.second.begin(), kv.second.end());
+            for (int c0 = 0; c0 < cnt; c0 += JRED_TASKS) {
+                JRedItem it;
+                it.jbase = L.jbase;
+                it.tfirst = L.first;
+                it.ket0 = kv.first;
+                it.nfc = ncart(L.lc);
+                it.nfd = ncart(L.ld);
+                it.lfirst = (int)item_tasks.size() - cnt + c0;
+                it.lcount = std::min(JRED_TASKS, cnt - c0);
+                items.push_back(it);
+                max_nfcd_ = std::max(max_nfcd_, it.nfc * it.nfd);
+            }
+        }
+        L.icount = (int)items.size() - L.ifirst;
+    }
+    nitems_ = (int)items.size();
+    dev_free(d_items_);
+    dev_free(d_item_tasks_);
+    d_items_ = dev_upload(items);
+    d_item_tasks_ = dev_upload(item_tasks);
+    const size_t need = jslab_whole_ ? total : largest;
+    if (need > nJ_) {
+        dev_free(d_J_);
+        check(hipMalloc(&d_J_, sizeof(double) * need), "hipMalloc");
+        nJ_ = need;
+    }
+    if (getenv("LIBRINT_DEBUG"))
+        fprintf(stderr, "librint: coulomb slabs %zu doubles (%s), %d reduction items\n", need,
+                jslab_whole_ ? "one reduction" : "per launch", nitems_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
     evc_.assign(launches_.size(), nullptr);
     for (auto &e : evc_) check(hipEventCreate(&e), "hipEventCreate");
@@ -519,16 +602,24 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_);
         const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
         L.variant = grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2);
+        double *js = L.jbase >= 0 ? d_J_ + L.jbase : nullptr;
         if (grouped)
             check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
-                                  gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, st),
+                                  gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, js, L.first, st),
                   "jk group launch");
         else
             check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + L.first, L.count, ctr,
-                            flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_, nao_, st),
+                            flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_, nao_, js, 0, st),
                   "jk launch");
+        if (!jslab_whole_ && L.icount)
+            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, 1, d_J_, B,
+                                 d_M_, nao_, st),
+                  "coulomb reduction launch");
         check(hipEventRecord(evc_[k], st), "hipEventRecord");
     }
+    if (jslab_whole_ && nitems_)
+        check(launch_jreduce(d_items_, nitems_, max_nfcd_, d_item_tasks_, d_tasks_, flags, 1, d_J_, B, d_M_, nao_, st),
+              "coulomb reduction launch");
     check(hipEventRecord(ev1_, st), "hipEventRecord");
     check(launch_symmetrize(d_M_, G_dev, nao_, st), "symmetrize launch");
 }

@@ -41,6 +41,8 @@ struct ClassLaunch {
     double flops = 0;
     float ms = 0.f;    // device time of this launch in the last build
     int variant = -1;  // kernel of the last build: 0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel
+    long long jbase = -1;  // Coulomb slab offset (-1: ket Coulomb block by global atomics)
+    int ifirst = 0, icount = 0;  // slab reduction items of this launch
 };
 
 class Context {
@@ -144,6 +146,14 @@ private:
     int max_lds_ = 0;
     bool no_group_ = false;      // LIBRINT_NO_GROUP: flat per-task kernels only
     double *d_M_ = nullptr, *d_D_ = nullptr, *d_G_ = nullptr;
+    // ket-side Coulomb slabs and their reduction items (kernels.h)
+    double *d_J_ = nullptr;
+    size_t nJ_ = 0;
+    JRedItem *d_items_ = nullptr;
+    int *d_item_tasks_ = nullptr;
+    int nitems_ = 0, max_nfcd_ = 1;
+    bool jslab_whole_ = true;     // one reduction after all launches (else one per launch)
+    bool no_jslab_ = false;       // LIBRINT_NO_JSLAB: global atomics only
     int2 *d_pp_idx_ = nullptr;      // adjoint kernel: primitive indices per record
     int2 *d_shell_slot_ = nullptr;  // {atom, first primitive slot} per shell
     double *d_slot_coef_ = nullptr; // contraction coefficient per slot

@@ -45,13 +45,29 @@ struct ListTask {
     long long out;
 };
 
+// Ket-side Coulomb slabs: instead of one global atomic per quartet into
+// M[c][d], a J/K launch stores each quartet's ket-pair Coulomb block in the
+// slab of its task, J[jbase + (t - tfirst)*64*NFCD + e*64 + lane]
+// (e = fc*NFD + fd, lane = ket - ket0), and the reduction sums the slabs of
+// every task that shares a 64-ket window (same ket0) before one atomic per
+// element.  One item per (launch, window).
+struct JRedItem {
+    long long jbase;  // slab offset of the launch
+    int tfirst;       // first task of the launch (task index space of item_tasks)
+    int ket0;         // first ket pair of the window
+    int nfc, nfd;     // ket shell sizes
+    int lfirst, lcount;  // range in item_tasks
+};
+
 bool class_supported(int la, int lb, int lc, int ld);
 // Task flags: 0 FP64, 1 FP32 recurrences, 2 skipped; a launch processes the
 // tasks whose flag bit is set in `accept` (flags == nullptr: all tasks).
 // counter: one int per launch, zero before the launch (launch_zero_counters)
+// jslab: slab base of this launch (null: ket Coulomb block by global atomics);
+// slab of task t = jslab + (t - tfirst)*64*NFC*NFD with t indexing `tasks`
 hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tasks, int ntask, int *counter,
                      const unsigned char *flags, int accept, const DevBasis &B, const double *D, double *M, int nao,
-                     hipStream_t st);
+                     double *jslab, int tfirst, hipStream_t st);
 hipError_t launch_zero_counters(int *p, int n, hipStream_t st);
 bool class_has_fp32(int la, int lb, int lc, int ld);
 // grouped form (exchange rows of the bra in LDS); jk_group_lds gives its LDS
@@ -59,7 +75,12 @@ bool class_has_fp32(int la, int lb, int lc, int ld);
 size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao);
 hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
                            const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
-                           int accept, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st);
+                           int accept, const DevBasis &B, const double *D, double *M, int nao, double *jslab,
+                           int tfirst, hipStream_t st);
+// sum the Coulomb slabs of every item's tasks into M (tasks: global task array)
+hipError_t launch_jreduce(const JRedItem *items, int nitem, int max_nfcd, const int *item_tasks, const JKTask *tasks,
+                          const unsigned char *flags, int accept, const double *J, const DevBasis &B, double *M,
+                          int nao, hipStream_t st);
 // per-build task classification (shell-block max |D|, then flags and group masks)
 hipError_t launch_classify(const JKTask *tasks, int ntask, const int *task_group, int ngroup, const DevBasis &B,
                            const double *D, const int *ao_off, int nsh, int nao, double *Dsh, double tau_mix,

@@ -116,7 +116,7 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
 
 template <int LA, int LB, int LC, int LD, typename T>
 __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, const double *__restrict__ D,
-                                        double *__restrict__ M, int nao, int lane, const T *rt)
+                                        double *__restrict__ M, int nao, int lane, const T *rt, double *jslab)
 {
     using C = Class<LA, LB, LC, LD>;
     const bool active = lane < t.nket;
@@ -155,7 +155,10 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
             for (int fb = 0; fb < C::NFB; fb++)
 #pragma unroll
                 for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + bra.aob + fb] * ERI_(fa, fb, fc, fd);
-            atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], 2.0 * s * acc);
+            if (jslab)
+                jslab[(fc * C::NFD + fd) * 64 + lane] = 2.0 * s * acc;
+            else
+                atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], 2.0 * s * acc);
         }
     const double h = -0.5 * s;
 #pragma unroll
@@ -209,8 +212,10 @@ __device__ __forceinline__ void jk_task(const JKTask &t, const DevBasis &B, cons
 template <int LA, int LB, int LC, int LD, typename T>
 __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ tasks, int ntask, int *__restrict__ counter,
                                                  const unsigned char *__restrict__ flags, int accept, DevBasis B,
-                                                 const double *__restrict__ D, double *__restrict__ M, int nao)
+                                                 const double *__restrict__ D, double *__restrict__ M, int nao,
+                                                 double *__restrict__ jslab, int tfirst)
 {
+    using C = Class<LA, LB, LC, LD>;
     using R = RysLds<LA, LB, LC, LD, T>;
     __shared__ T rt[R::LEN];
     rys_stage<T, R::N, R::WONLY>(rt);
@@ -222,7 +227,8 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
         t = __builtin_amdgcn_readfirstlane(t);
         if (t >= ntask || t < 0) break;
         if (flags && !((accept >> flags[t]) & 1)) continue;
-        jk_task<LA, LB, LC, LD, T>(tasks[t], B, D, M, nao, lane, rt);
+        double *js = jslab ? jslab + (size_t)(t - tfirst) * (64 * C::NFC * C::NFD) : nullptr;
+        jk_task<LA, LB, LC, LD, T>(tasks[t], B, D, M, nao, lane, rt, js);
     }
 }
 
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
 template <int LA, int LB, int LC, int LD, typename T>
 __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &bra, const DevBasis &B,
                                              const double *__restrict__ D, double *__restrict__ M, int nao, int lane,
-                                             const T *rt, double *Krow, double *mab)
+                                             const T *rt, double *Krow, double *mab, double *jslab)
 {
     using C = Class<LA, LB, LC, LD>;
     const bool active = lane < t.nket;
@@ -272,7 +278,10 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
             for (int fb = 0; fb < C::NFB; fb++)
 #pragma unroll
                 for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + bra.aob + fb] * ERI_(fa, fb, fc, fd);
-            atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], 2.0 * s * acc);
+            if (jslab)
+                jslab[(fc * C::NFD + fd) * 64 + lane] = 2.0 * s * acc;
+            else
+                atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], 2.0 * s * acc);
         }
     const double h = -0.5 * s;
     double *Ka = Krow, *Kb = Krow + C::NFA * n;
@@ -334,7 +343,8 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
                                                        const JKTask *__restrict__ tasks, int *__restrict__ counter,
                                                        const unsigned char *__restrict__ flags,
                                                        const unsigned char *__restrict__ gmask, int accept, DevBasis B,
-                                                       const double *__restrict__ D, double *__restrict__ M, int nao)
+                                                       const double *__restrict__ D, double *__restrict__ M, int nao,
+                                                       double *__restrict__ jslab, int tfirst)
 {
     using C = Class<LA, LB, LC, LD>;
     using R = RysLds<LA, LB, LC, LD, T>;
@@ -369,7 +379,8 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
         for (int e = 0; e < C::NFA * C::NFB; e++) mab[e] = 0.0;
         for (int t = G.first + wave; t < G.first + G.count; t += 4) {
             if (flags && !((accept >> flags[t]) & 1)) continue;
-            jk_task_rows<LA, LB, LC, LD, T>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab);
+            double *js = jslab ? jslab + (size_t)(t - tfirst) * (64 * C::NFC * C::NFD) : nullptr;
+            jk_task_rows<LA, LB, LC, LD, T>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab, js);
         }
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) {
@@ -401,7 +412,8 @@ static size_t group_lds_bytes(int nao)
 template <int LA, int LB, int LC, int LD, typename T>
 static hipError_t launch_jk_group_t(const JKGroup *groups, int ngroup, const JKTask *tasks, int *counter,
                                     const unsigned char *flags, const unsigned char *gmask, int accept,
-                                    const DevBasis &B, const double *D, double *M, int nao, hipStream_t st)
+                                    const DevBasis &B, const double *D, double *M, int nao, double *jslab,
+                                    int tfirst, hipStream_t st)
 {
     const size_t lds = group_lds_bytes<LA, LB, LC, LD, T>(nao);
     static int grid = 0, grid_nao = -1;
@@ -418,7 +430,7 @@ static hipError_t launch_jk_group_t(const JKGroup *groups, int ngroup, const JKT
     }
     const int g = std::min(grid, ngroup);
     hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, T>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks, counter,
-                       flags, gmask, accept, B, D, M, nao);
+                       flags, gmask, accept, B, D, M, nao, jslab, tfirst);
     return hipGetLastError();
 }
 
@@ -872,7 +884,7 @@ bool class_supported(int la, int lb, int lc, int ld)
 template <int LA, int LB, int LC, int LD, typename T>
 static hipError_t launch_jk_persistent(const JKTask *tasks, int ntask, int *counter, const unsigned char *flags,
                                        int accept, const DevBasis &B, const double *D, double *M, int nao,
-                                       hipStream_t st)
+                                       double *jslab, int tfirst, hipStream_t st)
 {
     static int grid = 0;
     if (grid == 0) {
@@ -887,7 +899,7 @@ static hipError_t launch_jk_persistent(const JKTask *tasks, int ntask, int *coun
     }
     const int g = std::min(grid, (ntask + 3) / 4);
     hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, T>), dim3(g), dim3(256), 0, st, tasks, ntask, counter, flags, accept, B,
-                       D, M, nao);
+                       D, M, nao, jslab, tfirst);
     return hipGetLastError();
 }
 
@@ -915,16 +927,18 @@ bool class_has_fp32(int la, int lb, int lc, int ld)
 
 hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tasks, int ntask, int *counter,
                      const unsigned char *flags, int accept, const DevBasis &B, const double *D, double *M, int nao,
-                     hipStream_t st)
+                     double *jslab, int tfirst, hipStream_t st)
 {
     if (ntask == 0) return hipSuccess;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
-        return fp32 ? launch_jk_persistent<a, b, c, d, float>(tasks, ntask, counter, flags, accept, B, D, M, nao, st)  \
-                    : launch_jk_persistent<a, b, c, d, double>(tasks, ntask, counter, flags, accept, B, D, M, nao, st);
+        return fp32 ? launch_jk_persistent<a, b, c, d, float>(tasks, ntask, counter, flags, accept, B, D, M, nao,       \
+                                                              jslab, tfirst, st)                                       \
+                    : launch_jk_persistent<a, b, c, d, double>(tasks, ntask, counter, flags, accept, B, D, M, nao,      \
+                                                               jslab, tfirst, st);
     LR_REG_CLASSES(X)
 #undef X
-    if (fp32) return hipErrorInvalidValue;
+    if (fp32 || jslab) return hipErrorInvalidValue;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d) {                                                                    \
         hipLaunchKernelGGL((jk_team_kernel<a, b, c, d>), dim3(ntask), dim3(64), 0, st, tasks, flags, accept, B, D, M,   \
@@ -948,15 +962,16 @@ size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao)
 
 hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
                            const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
-                           int accept, const DevBasis &B, const double *D, double *M, int nao, hipStream_t st)
+                           int accept, const DevBasis &B, const double *D, double *M, int nao, double *jslab,
+                           int tfirst, hipStream_t st)
 {
     if (ngroup == 0) return hipSuccess;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
         return fp32 ? launch_jk_group_t<a, b, c, d, float>(groups, ngroup, tasks, counter, flags, gmask, accept, B, D, \
-                                                           M, nao, st)                                                 \
+                                                           M, nao, jslab, tfirst, st)                                  \
                     : launch_jk_group_t<a, b, c, d, double>(groups, ngroup, tasks, counter, flags, gmask, accept, B,   \
-                                                            D, M, nao, st);
+                                                            D, M, nao, jslab, tfirst, st);
     LR_REG_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -984,6 +999,59 @@ hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, in
     LR_LIST_TEAM_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+// One workgroup of 4 wavefronts per (item, ket component e): lane = ket offset
+// in the window; the wavefronts split the item's tasks, each summing the
+// slab entries of the executed tasks whose ket range covers the lane; the
+// partial sums are combined in LDS and added to M[c][d] with one atomic.
+__global__ __launch_bounds__(256) void jreduce_kernel(const JRedItem *__restrict__ items, int nitem,
+                                                      const int *__restrict__ item_tasks,
+                                                      const JKTask *__restrict__ tasks,
+                                                      const unsigned char *__restrict__ flags, int accept,
+                                                      const double *__restrict__ J, DevBasis B,
+                                                      double *__restrict__ M, int nao)
+{
+    __shared__ double part[4][64];
+    __shared__ int seen[4][64];
+    const int ii = blockIdx.x;
+    if (ii >= nitem) return;
+    const JRedItem it = items[ii];
+    const int nfcd = it.nfc * it.nfd;
+    const int e = blockIdx.y;
+    if (e >= nfcd) return;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double acc = 0.0;
+    int any = 0;
+    for (int k = wave; k < it.lcount; k += 4) {
+        const int t = item_tasks[it.lfirst + k];
+        if (flags && !((accept >> flags[t]) & 1)) continue;
+        if (lane < tasks[t].nket) {
+            acc += J[it.jbase + (long long)(t - it.tfirst) * (64 * nfcd) + e * 64 + lane];
+            any = 1;
+        }
+    }
+    part[wave][lane] = acc;
+    seen[wave][lane] = any;
+    __syncthreads();
+    if (wave != 0) return;
+    acc = part[0][lane] + part[1][lane] + part[2][lane] + part[3][lane];
+    any = seen[0][lane] | seen[1][lane] | seen[2][lane] | seen[3][lane];
+    if (!any) return;
+    const int4 pi = B.pair_info[it.ket0 + lane];
+    const int fc = e / it.nfd, fd = e - fc * it.nfd;
+    atomicAdd(&M[(size_t)(B.ao_off[pi.x] + fc) * nao + B.ao_off[pi.y] + fd], acc);
+}
+
+hipError_t launch_jreduce(const JRedItem *items, int nitem, int max_nfcd, const int *item_tasks, const JKTask *tasks,
+                          const unsigned char *flags, int accept, const double *J, const DevBasis &B, double *M,
+                          int nao, hipStream_t st)
+{
+    if (nitem <= 0) return hipSuccess;
+    if (max_nfcd < 1 || max_nfcd > 36) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(jreduce_kernel, dim3(nitem, max_nfcd), dim3(256), 0, st, items, nitem, item_tasks, tasks, flags,
+                       accept, J, B, M, nao);
+    return hipGetLastError();
 }
 
 hipError_t launch_symmetrize(const double *M, double *G, int nao, hipStream_t st)
