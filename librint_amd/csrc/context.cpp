@@ -91,6 +91,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     nao_ = ao;
     no_group_ = getenv("LIBRINT_NO_GROUP") != nullptr;
     no_jslab_ = getenv("LIBRINT_NO_JSLAB") != nullptr;
+    if (const char *e = getenv("LIBRINT_DROW")) drow_mode_ = atoi(e) != 0 ? 1 : 0;
     check(hipEventCreate(&ev0_), "hipEventCreate");
     check(hipEventCreate(&ev1_), "hipEventCreate");
     build_pairs();
@@ -452,7 +453,11 @@ void Context::build_tasks()
     // of the launch's tasks on that window.
     std::vector<JRedItem> items;
     std::vector<int> item_tasks;
-    const size_t JBUF_MAX = (size_t)1 << 28;  // doubles (2 GiB)
+    // slab budget in doubles: a launch above JBUF_LAUNCH keeps the global
+    // atomics; one reduction after the whole build while the total stays
+    // under JBUF_WHOLE, else the buffer holds one launch at a time
+    const size_t JBUF_LAUNCH = (size_t)1 << 30;  // 8 GiB
+    const size_t JBUF_WHOLE = (size_t)1 << 28;   // 2 GiB
     size_t total = 0, largest = 0;
     max_nfcd_ = 1;
     for (ClassLaunch &L : launches_) {
@@ -460,12 +465,12 @@ void Context::build_tasks()
         L.ifirst = L.icount = 0;
         if (no_jslab_ || !class_has_fp32(L.la, L.lb, L.lc, L.ld)) continue;
         const size_t sz = (size_t)L.count * 64 * ncart(L.lc) * ncart(L.ld);
-        if (sz > JBUF_MAX) continue;
+        if (sz > JBUF_LAUNCH) continue;
         L.jbase = (long long)total;
         total += sz;
         largest = std::max(largest, sz);
     }
-    jslab_whole_ = total <= JBUF_MAX;
+    jslab_whole_ = total <= JBUF_WHOLE;
     for (ClassLaunch &L : launches_) {
         if (L.jbase < 0) continue;
         if (!jslab_whole_) L.jbase = 0;
@@ -599,13 +604,20 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     for (size_t k = 0; k < launches_.size(); k++) {
         ClassLaunch &L = launches_[k];
         int *ctr = d_counters_ + 2 * k;
-        const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_);
+        const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, false);
         const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
+        // staging the bra's density rows in LDS (LIBRINT_DROW=1) halves the
+        // row space per workgroup; measured slower overall on a 992-function
+        // cluster, so it is off by default
+        bool drow = drow_mode_ == 1;
+        drow = drow && grouped && jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, true) <= (size_t)max_lds_;
+        L.drow = drow ? 1 : 0;
         L.variant = grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2);
         double *js = L.jbase >= 0 ? d_J_ + L.jbase : nullptr;
         if (grouped)
             check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
-                                  gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, js, L.first, st),
+                                  gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, js, L.first, drow,
+                                  st),
                   "jk group launch");
         else
             check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + L.first, L.count, ctr,

@@ -43,6 +43,7 @@ struct ClassLaunch {
     int variant = -1;  // kernel of the last build: 0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel
     long long jbase = -1;  // Coulomb slab offset (-1: ket Coulomb block by global atomics)
     int ifirst = 0, icount = 0;  // slab reduction items of this launch
+    int drow = 0;                // bra density rows staged in LDS in the last build
 };
 
 class Context {
@@ -154,6 +155,7 @@ private:
     int nitems_ = 0, max_nfcd_ = 1;
     bool jslab_whole_ = true;     // one reduction after all launches (else one per launch)
     bool no_jslab_ = false;       // LIBRINT_NO_JSLAB: global atomics only
+    int drow_mode_ = 0;           // LIBRINT_DROW=1: stage the bra density rows in LDS
     int2 *d_pp_idx_ = nullptr;      // adjoint kernel: primitive indices per record
     int2 *d_shell_slot_ = nullptr;  // {atom, first primitive slot} per shell
     double *d_slot_coef_ = nullptr; // contraction coefficient per slot

@@ -72,11 +72,12 @@ hipError_t launch_zero_counters(int *p, int n, hipStream_t st);
 bool class_has_fp32(int la, int lb, int lc, int ld);
 // grouped form (exchange rows of the bra in LDS); jk_group_lds gives its LDS
 // bytes per workgroup, (size_t)-1 for classes without a grouped kernel
-size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao);
+// (drow: with the bra's density rows staged in LDS as well)
+size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao, bool drow);
 hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
                            const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
                            int accept, const DevBasis &B, const double *D, double *M, int nao, double *jslab,
-                           int tfirst, hipStream_t st);
+                           int tfirst, bool drow, hipStream_t st);
 // sum the Coulomb slabs of every item's tasks into M (tasks: global task array)
 hipError_t launch_jreduce(const JRedItem *items, int nitem, int max_nfcd, const int *item_tasks, const JKTask *tasks,
                           const unsigned char *flags, int accept, const double *J, const DevBasis &B, double *M,

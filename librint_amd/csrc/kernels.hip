@@ -240,10 +240,11 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
 // flushed once per group with coalesced global atomics; the bra-bra Coulomb
 // block is reduced over the whole group.  Only the ket-ket Coulomb block
 // (M[c][d], one atomic per element per quartet) goes straight to memory.
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, typename T, bool DROW>
 __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &bra, const DevBasis &B,
                                              const double *__restrict__ D, double *__restrict__ M, int nao, int lane,
-                                             const T *rt, double *Krow, double *mab, double *jslab)
+                                             const T *rt, double *Krow, double *mab, double *jslab,
+                                             const double *Drow)
 {
     using C = Class<LA, LB, LC, LD>;
     const bool active = lane < t.nket;
@@ -285,6 +286,11 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
         }
     const double h = -0.5 * s;
     double *Ka = Krow, *Kb = Krow + C::NFA * n;
+    // density rows a and b: from LDS (DROW) or from D
+    auto Da = [&](int fa, size_t col) { return DROW ? Drow[fa * n + col] : D[(bra.aoa + fa) * n + col]; };
+    auto Db = [&](int fb, size_t col) {
+        return DROW ? Drow[(C::NFA + fb) * n + col] : D[(bra.aob + fb) * n + col];
+    };
 #pragma unroll
     for (int fc = 0; fc < C::NFC; fc++) {
 #pragma unroll
@@ -293,7 +299,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fb = 0; fb < C::NFB; fb++) acc += D[(bra.aob + fb) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
+                for (int fb = 0; fb < C::NFB; fb++) acc += Db(fb, ket.aob + fd) * ERI_(fa, fb, fc, fd);
             atomicAdd(&Ka[fa * n + ket.aoa + fc], h * acc);
         }
 #pragma unroll
@@ -302,7 +308,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fd = 0; fd < C::NFD; fd++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
+                for (int fa = 0; fa < C::NFA; fa++) acc += Da(fa, ket.aob + fd) * ERI_(fa, fb, fc, fd);
             atomicAdd(&Kb[fb * n + ket.aoa + fc], h * acc);
         }
     }
@@ -314,7 +320,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
-                for (int fb = 0; fb < C::NFB; fb++) acc += D[(bra.aob + fb) * n + ket.aoa + fc] * ERI_(fa, fb, fc, fd);
+                for (int fb = 0; fb < C::NFB; fb++) acc += Db(fb, ket.aoa + fc) * ERI_(fa, fb, fc, fd);
             atomicAdd(&Ka[fa * n + ket.aob + fd], h * acc);
         }
 #pragma unroll
@@ -323,7 +329,7 @@ __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &br
 #pragma unroll
             for (int fc = 0; fc < C::NFC; fc++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++) acc += D[(bra.aoa + fa) * n + ket.aoa + fc] * ERI_(fa, fb, fc, fd);
+                for (int fa = 0; fa < C::NFA; fa++) acc += Da(fa, ket.aoa + fc) * ERI_(fa, fb, fc, fd);
             atomicAdd(&Kb[fb * n + ket.aob + fd], h * acc);
         }
     }
@@ -337,8 +343,10 @@ __host__ __device__ constexpr int RysLdsDoubles()
     return (int)((RysLds<LA, LB, LC, LD, T>::LEN * sizeof(T) + 15) / 16 * 2);
 }
 
-// gmask: per group, bit f set when the group holds a task with flag f
-template <int LA, int LB, int LC, int LD, typename T>
+// gmask: per group, bit f set when the group holds a task with flag f.
+// DROW: the bra's density rows (NFA + NFB rows of nao) are staged in LDS
+// beside the exchange rows, so the exchange terms read D from LDS.
+template <int LA, int LB, int LC, int LD, typename T, bool DROW>
 __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
                                                        const JKTask *__restrict__ tasks, int *__restrict__ counter,
                                                        const unsigned char *__restrict__ flags,
@@ -352,6 +360,7 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
     T *rt = (T *)lds;                                       // Rys table block (16-byte aligned)
     double *mab_l = lds + RysLdsDoubles<LA, LB, LC, LD, T>(); // bra-bra block
     double *Krow = mab_l + C::NFA * C::NFB;                 // (NFA + NFB) rows of nao
+    double *Drow = Krow + (C::NFA + C::NFB) * nao;          // density rows (DROW)
     __shared__ int g_s;
     rys_stage<T, R::N, R::WONLY>(rt);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -374,13 +383,21 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
         if (g >= ngroup || g < 0) break;
         const JKGroup G = groups[g];
         const PairView bra = load_pair(B, G.bra);
+        if constexpr (DROW) {
+            for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
+                const int r = i / nao, col = i - r * nao;
+                const size_t row = (r < C::NFA) ? (size_t)(bra.aoa + r) : (size_t)(bra.aob + r - C::NFA);
+                Drow[i] = D[row * n + col];
+            }
+            __syncthreads();
+        }
         double mab[C::NFA * C::NFB];
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) mab[e] = 0.0;
         for (int t = G.first + wave; t < G.first + G.count; t += 4) {
             if (flags && !((accept >> flags[t]) & 1)) continue;
             double *js = jslab ? jslab + (size_t)(t - tfirst) * (64 * C::NFC * C::NFD) : nullptr;
-            jk_task_rows<LA, LB, LC, LD, T>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab, js);
+            jk_task_rows<LA, LB, LC, LD, T, DROW>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab, js, Drow);
         }
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) {
@@ -403,35 +420,48 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
 }
 
 template <int LA, int LB, int LC, int LD, typename T>
-static size_t group_lds_bytes(int nao)
+static size_t group_lds_bytes(int nao, bool drow)
 {
     using C = Class<LA, LB, LC, LD>;
-    return sizeof(double) * ((size_t)RysLdsDoubles<LA, LB, LC, LD, T>() + C::NFA * C::NFB + (size_t)(C::NFA + C::NFB) * nao);
+    return sizeof(double) * ((size_t)RysLdsDoubles<LA, LB, LC, LD, T>() + C::NFA * C::NFB +
+                             (size_t)(C::NFA + C::NFB) * nao * (drow ? 2 : 1));
+}
+
+template <int LA, int LB, int LC, int LD, typename T, bool DROW>
+static hipError_t launch_jk_group_d(const JKGroup *groups, int ngroup, const JKTask *tasks, int *counter,
+                                    const unsigned char *flags, const unsigned char *gmask, int accept,
+                                    const DevBasis &B, const double *D, double *M, int nao, double *jslab,
+                                    int tfirst, hipStream_t st)
+{
+    const size_t lds = group_lds_bytes<LA, LB, LC, LD, T>(nao, DROW);
+    static int grid = 0, grid_nao = -1;
+    if (grid == 0 || grid_nao != nao) {
+        int dev = 0, ncu = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_group_kernel<LA, LB, LC, LD, T, DROW>, 256, lds);
+        grid = std::max(1, ncu) * std::max(1, per);
+        grid_nao = nao;
+        if (getenv("LIBRINT_DEBUG"))
+            fprintf(stderr, "librint: jk_group<%d%d|%d%d>%s%s lds %zu blocks/cu %d grid %d\n", LA, LB, LC, LD,
+                    sizeof(T) == 4 ? " fp32" : "", DROW ? " drow" : "", lds, per, grid);
+    }
+    const int g = std::min(grid, ngroup);
+    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, T, DROW>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks,
+                       counter, flags, gmask, accept, B, D, M, nao, jslab, tfirst);
+    return hipGetLastError();
 }
 
 template <int LA, int LB, int LC, int LD, typename T>
 static hipError_t launch_jk_group_t(const JKGroup *groups, int ngroup, const JKTask *tasks, int *counter,
                                     const unsigned char *flags, const unsigned char *gmask, int accept,
                                     const DevBasis &B, const double *D, double *M, int nao, double *jslab,
-                                    int tfirst, hipStream_t st)
+                                    int tfirst, bool drow, hipStream_t st)
 {
-    const size_t lds = group_lds_bytes<LA, LB, LC, LD, T>(nao);
-    static int grid = 0, grid_nao = -1;
-    if (grid == 0 || grid_nao != nao) {
-        int dev = 0, ncu = 0, per = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_group_kernel<LA, LB, LC, LD, T>, 256, lds);
-        grid = std::max(1, ncu) * std::max(1, per);
-        grid_nao = nao;
-        if (getenv("LIBRINT_DEBUG"))
-            fprintf(stderr, "librint: jk_group<%d%d|%d%d>%s lds %zu blocks/cu %d grid %d\n", LA, LB, LC, LD,
-                    sizeof(T) == 4 ? " fp32" : "", lds, per, grid);
-    }
-    const int g = std::min(grid, ngroup);
-    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, T>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks, counter,
-                       flags, gmask, accept, B, D, M, nao, jslab, tfirst);
-    return hipGetLastError();
+    return drow ? launch_jk_group_d<LA, LB, LC, LD, T, true>(groups, ngroup, tasks, counter, flags, gmask, accept, B,
+                                                              D, M, nao, jslab, tfirst, st)
+                : launch_jk_group_d<LA, LB, LC, LD, T, false>(groups, ngroup, tasks, counter, flags, gmask, accept, B,
+                                                               D, M, nao, jslab, tfirst, st);
 }
 
 // ------------------------------------------------------ team J/K kernel --
@@ -950,11 +980,11 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tas
     return hipErrorInvalidValue;
 }
 
-size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao)
+size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao, bool drow)
 {
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
-        return fp32 ? group_lds_bytes<a, b, c, d, float>(nao) : group_lds_bytes<a, b, c, d, double>(nao);
+        return fp32 ? group_lds_bytes<a, b, c, d, float>(nao, drow) : group_lds_bytes<a, b, c, d, double>(nao, drow);
     LR_REG_CLASSES(X)
 #undef X
     return (size_t)-1;
@@ -963,15 +993,15 @@ size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao)
 hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
                            const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
                            int accept, const DevBasis &B, const double *D, double *M, int nao, double *jslab,
-                           int tfirst, hipStream_t st)
+                           int tfirst, bool drow, hipStream_t st)
 {
     if (ngroup == 0) return hipSuccess;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
         return fp32 ? launch_jk_group_t<a, b, c, d, float>(groups, ngroup, tasks, counter, flags, gmask, accept, B, D, \
-                                                           M, nao, jslab, tfirst, st)                                  \
+                                                           M, nao, jslab, tfirst, drow, st)                            \
                     : launch_jk_group_t<a, b, c, d, double>(groups, ngroup, tasks, counter, flags, gmask, accept, B,   \
-                                                            D, M, nao, jslab, tfirst, st);
+                                                            D, M, nao, jslab, tfirst, drow, st);
     LR_REG_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
