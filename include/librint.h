@@ -130,11 +130,12 @@ int librint_schwarz(librint_ctx *ctx, double *out);
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 
-/* Per angular-class launch of the current plan, 12 values each:
+/* Per angular-class launch of the current plan, 13 values each:
  * la, lb, lc, ld, quartets, primitive quartets, algorithmic flops, lanes
  * launched (64 x tasks), device ms of the launch in the last host-side build
  * (librint_fock), task count, FP32 recurrences (0/1), kernel of the last
- * build (0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, -1 none yet).
+ * build (0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, -1 none yet),
+ * bra density rows staged in LDS (0/1).
  * Returns the number of launches (out may be NULL to query it); writes at
  * most n launches. */
 int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);

@@ -541,10 +541,11 @@ int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n)
     int k = 0;
     for (; k < (int)L.size() && k < n; k++) {
         const ClassLaunch &c = L[k];
-        const double v[12] = {(double)c.la,        (double)c.lb,    (double)c.lc,      (double)c.ld,
+        const double v[13] = {(double)c.la,        (double)c.lb,    (double)c.lc,      (double)c.ld,
                               (double)c.quartets,  (double)c.prim_quartets, c.flops,  (double)c.lanes,
-                              (double)c.ms,        (double)c.count, (double)c.fp32,    (double)c.variant};
-        for (int j = 0; j < 12; j++) out[12 * k + j] = v[j];
+                              (double)c.ms,        (double)c.count, (double)c.fp32,    (double)c.variant,
+                              (double)c.drow};
+        for (int j = 0; j < 13; j++) out[13 * k + j] = v[j];
     }
     return (int)L.size();
 }
