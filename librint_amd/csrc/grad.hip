@@ -12,9 +12,10 @@
 //   d/dA_x      = 2 alpha G(a + 1_x) - a_x G(a - 1_x)
 //   d/dalpha    = -(G(a + 2_x) + G(a + 2_y) + G(a + 2_z))
 //   d/dc        = (primitive part of the contracted integral) / c
-// The Rys 1D integrals are built with the bra (SIDE 0: centres a, b) or the
-// ket (SIDE 1: centres c, d) momentum raised by 2; every derivative swaps
-// one axis factor of sum_r Ix Iy Iz for its shifted combination.
+// The Rys 1D integrals are built with the momentum of the differentiated
+// pair (SIDE 0: bra centres a, b; SIDE 1: ket centres c, d, computed as the
+// bra of (cd|ab)) raised by 2; every derivative swaps one axis factor of
+// sum_r Ix Iy Iz for its shifted combination.
 //
 // Work mapping: a team of TS lanes owns one quartet; lanes c < 3*NROOTS
 // build the (root, axis) chains of the raised recurrence in LDS, then every
@@ -219,7 +220,14 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
                                                   const double *__restrict__ coef, const double *__restrict__ D, int nao,
                                                   double *__restrict__ gatom, double *__restrict__ dprim)
 {
-    using S = GShape<LA, LB, LC, LD, SIDE>;
+    // The differentiated pair X (bra for SIDE 0, ket for SIDE 1) always takes
+    // the bra role of the raised recurrence: the ket side is computed as the
+    // bra side of (cd|ab), which has the same energy weights, so both passes
+    // raise the momentum along the unrolled n direction of the VRR.
+    constexpr int XA = SIDE == 0 ? LA : LC, XB = SIDE == 0 ? LB : LD;
+    constexpr int YA = SIDE == 0 ? LC : LA, YB = SIDE == 0 ? LD : LB;
+    using S = GShape<XA, XB, YA, YB, 0>;
+    using C = Class<LA, LB, LC, LD>;
     __shared__ double g[S::NT * S::NCH * S::GS];
     const int tid = blockIdx.x;
     if (tid >= ntask) return;
@@ -250,26 +258,27 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
             gi[k] = 0;
             pw[k] = 0;
             if (f < S::NF) {
-                const int fa = f % S::NFA, t1 = f / S::NFA;
-                const int fb = t1 % S::NFB, t2 = t1 / S::NFB;
-                const int fc = t2 % S::NFC, fd = t2 / S::NFC;
+                const int fa = f % C::NFA, t1 = f / C::NFA;
+                const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
+                const int fc = t2 % C::NFC, fd = t2 / C::NFC;
                 const size_t a = bra.aoa + fa, b = bra.aob + fb, c = ket.aoa + fc, d = ket.aob + fd;
                 W[k] = s * (4.0 * D[a * n + b] * D[c * n + d] - D[a * n + c] * D[b * n + d] - D[a * n + d] * D[b * n + c]);
                 int packed = 0, pws = 0;
 #pragma unroll
                 for (int x = 0; x < 3; x++) {
                     const int ia = GCART[LA][fa][x], ib = GCART[LB][fb][x], ic = GCART[LC][fc][x], id = GCART[LD][fd][x];
-                    packed |= (ia + ic * S::DK + id * S::DL + ib * S::DJ) << (10 * x);
-                    const int p0 = SIDE == 0 ? ia : ic, p1 = SIDE == 0 ? ib : id;
-                    pws |= (p0 << (2 * x)) | (p1 << (6 + 2 * x));
+                    const int xa = SIDE == 0 ? ia : ic, xb = SIDE == 0 ? ib : id;
+                    const int ya = SIDE == 0 ? ic : ia, yb = SIDE == 0 ? id : ib;
+                    packed |= (xa + ya * S::DK + yb * S::DL + xb * S::DJ) << (10 * x);
+                    pws |= (xa << (2 * x)) | (xb << (6 + 2 * x));
                 }
                 gi[k] = packed;
                 pw[k] = pws;
             }
         }
-        const double rbd[3] = {bra.rb[0] - ket.rb[0], bra.rb[1] - ket.rb[1], bra.rb[2] - ket.rb[2]};
         const GPair &sd = SIDE == 0 ? bra : ket;  // pair whose centres are differentiated
         const GPair &ot = SIDE == 0 ? ket : bra;
+        const double rxy[3] = {sd.rb[0] - ot.rb[0], sd.rb[1] - ot.rb[1], sd.rb[2] - ot.rb[2]};
         const int2 si0 = shell_info[sd.sa], si1 = shell_info[sd.sb];
         double gx0[3] = {0.0, 0.0, 0.0}, gx1[3] = {0.0, 0.0, 0.0};
         for (int o = 0; o < sd.npp; o++) {
@@ -278,12 +287,7 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
             double e0 = 0.0, e1 = 0.0, val = 0.0;
             for (int i = 0; i < ot.npp; i++) {
                 const PrimPair pi = B.pp[ot.off + i];
-                if (tl < S::NCH) {
-                    if constexpr (SIDE == 0)
-                        grad_chain<LA, LB, LC, LD, SIDE>(gt, tl, po, pi, rbd, bra.rab, ket.rab);
-                    else
-                        grad_chain<LA, LB, LC, LD, SIDE>(gt, tl, pi, po, rbd, bra.rab, ket.rab);
-                }
+                if (tl < S::NCH) grad_chain<XA, XB, YA, YB, 0>(gt, tl, po, pi, rxy, sd.rab, ot.rab);
                 __syncthreads();
 #pragma unroll
                 for (int k = 0; k < S::NPL; k++) {
