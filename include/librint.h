@@ -21,6 +21,7 @@
 #define LIBRINT_AMD_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -161,6 +162,38 @@ double *grad_nuc_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, 
 
 double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess, int diis, double *P_inout,
                    int *iters);
+
+/* librint_scf with an energy criterion as well: converged when
+ * |P - P_old|_F / 2 <= conv and (conv_e <= 0 or |E_k - E_(k-1)| <= conv_e),
+ * E_k the energy of the density entering iteration k.  No counterpart in the
+ * reference. */
+double librint_scf_ex(librint_ctx *ctx, int nelec, int imax, double conv, double conv_e, int guess, int diis,
+                      double *P_inout, int *iters);
+
+/* ------------------------------------------------ learned SCF guess ---- */
+/* Kernels of the density-guess MLP (BF16 weights/activations as raw uint16
+ * patterns, FP32 accumulation).  All array arguments are device pointers;
+ * stream NULL = the default stream.  Return 0, or -1 with librint_last_error.
+ * No counterpart in the reference (its SCF starts from P = I).
+ *   gemm:     C = act(op(A) op(B) + bias [+ C]); op(A)(m,k) = transA ? A[k*lda+m]
+ *             : A[m*lda+k], op(B)(k,n) = transB ? B[n*ldb+k] : B[k*ldb+n];
+ *             act 0 none, 1 ReLU; C (FP32) and/or Cb (BF16) may be NULL (not both)
+ *   relu_bwd: dpre = dz * (h > 0) as FP32 and/or BF16
+ *   colsum:   out[c] (+)= sum_r x[r*ld + c]
+ *   mse:      dy = 2 mask (y - t) / scale on the first cols columns (0 beyond),
+ *             *loss += sum mask (y - t)^2 / scale
+ *   adam:     Adam step (bias-corrected, step >= 1) on FP32 w, refreshing wb
+ *   to_bf16:  y = bf16(x), round to nearest even */
+int librint_mlp_gemm(int M, int N, int K, const uint16_t *A, int lda, int transA, const uint16_t *B, int ldb,
+                     int transB, float *C, int ldc, const float *bias, int act, int accumulate, uint16_t *Cb, int ldcb,
+                     void *stream);
+int librint_mlp_relu_bwd(size_t n, const float *dz, const float *h, float *dpre, uint16_t *dpre_b, void *stream);
+int librint_mlp_colsum(int rows, int cols, const float *x, int ld, float *out, int accumulate, void *stream);
+int librint_mlp_mse(int rows, int cols, const float *y, int ldy, const float *t, const float *mask, int ldt,
+                    float scale, float *dy, uint16_t *dy_b, double *loss, void *stream);
+int librint_mlp_adam(size_t n, float *w, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                     int step, uint16_t *wb, void *stream);
+int librint_mlp_to_bf16(size_t n, const float *x, uint16_t *y, void *stream);
 
 /* Last error message of this thread ("" if none). */
 const char *librint_last_error(void);

@@ -74,6 +74,21 @@ library.librint_ctx_gradient.argtypes = (ctypes.c_void_p, _dp, _dp, _dp, _dp)
 library.librint_ctx_gradient.restype = ctypes.c_int
 library.grad_nuc_c.argtypes = _mol + (_dp, _sz)
 library.grad_nuc_c.restype = _dp
+library.librint_scf_ex.argtypes = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_int, _dp, _ip)
+library.librint_scf_ex.restype = ctypes.c_double
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+library.librint_mlp_gemm.argtypes = (_i, _i, _i, _vp, _i, _i, _vp, _i, _i, _vp, _i, _vp, _i, _i, _vp, _i, _vp)
+library.librint_mlp_relu_bwd.argtypes = (_sz, _vp, _vp, _vp, _vp, _vp)
+library.librint_mlp_colsum.argtypes = (_i, _i, _vp, _i, _vp, _i, _vp)
+library.librint_mlp_mse.argtypes = (_i, _i, _vp, _i, _vp, _vp, _i, _f, _vp, _vp, _vp, _vp)
+library.librint_mlp_adam.argtypes = (_sz, _vp, _vp, _vp, _vp, _f, _f, _f, _f, _i, _vp, _vp)
+library.librint_mlp_to_bf16.argtypes = (_sz, _vp, _vp, _vp)
+for _name in ("librint_mlp_gemm", "librint_mlp_relu_bwd", "librint_mlp_colsum", "librint_mlp_mse",
+              "librint_mlp_adam", "librint_mlp_to_bf16"):
+    getattr(library, _name).restype = ctypes.c_int
 library.librint_last_error.argtypes = ()
 library.librint_last_error.restype = ctypes.c_char_p
 
@@ -84,5 +99,7 @@ EXPORTED = (
     "librint_ctx_destroy", "librint_ctx_nao", "librint_ctx_set_schwarz", "librint_fock",
     "librint_fock_device", "librint_int2e_shells", "librint_schwarz", "librint_ctx_stats",
     "librint_ctx_class_stats", "librint_ctx_set_precision", "librint_ctx_flag_counts",
-    "librint_scf", "librint_last_error", "librint_ctx_gradient", "grad_nuc_c",
+    "librint_scf", "librint_last_error", "librint_ctx_gradient", "grad_nuc_c", "librint_scf_ex",
+    "librint_mlp_gemm", "librint_mlp_relu_bwd", "librint_mlp_colsum", "librint_mlp_mse", "librint_mlp_adam",
+    "librint_mlp_to_bf16",
 )

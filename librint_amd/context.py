@@ -112,9 +112,12 @@ class FockContext:
                 r["kernel"] = None
         return rows
 
-    def scf(self, imax=200, conv=1e-6, guess=0, diis=False, P0=None):
+    def scf(self, imax=200, conv=1e-6, guess=0, diis=False, P0=None, conv_e=0.0):
+        """(E, P, iterations); guess 0 identity, 1 core Hamiltonian, 2 P0.
+        conv_e > 0 adds |E_k - E_(k-1)| <= conv_e to the density criterion."""
         P = np.zeros((self.nao, self.nao)) if P0 is None else np.ascontiguousarray(P0, dtype=np.float64).copy()
         it = ctypes.c_int(0)
-        E = library.librint_scf(self.h, int(self.nelectron), int(imax), float(conv), int(guess), int(bool(diis)),
-                                P.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(it))
+        E = library.librint_scf_ex(self.h, int(self.nelectron), int(imax), float(conv), float(conv_e), int(guess),
+                                   int(bool(diis)), P.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   ctypes.byref(it))
         return E, P, it.value

@@ -12,6 +12,7 @@
 #include "context.h"
 #include "int1e.h"
 #include "linalg.h"
+#include "mlp.h"
 #include "scf.h"
 
 using namespace librint;
@@ -580,6 +581,33 @@ int librint_ctx_gradient(librint_ctx *ctx, const double *P, double *gnuc, double
     }
 }
 
+double librint_scf_ex(librint_ctx *ctx, int nelec, int imax, double conv, double conv_e, int guess, int diis,
+                      double *P_inout, int *iters)
+{
+    try {
+        if (!ctx) {
+            set_err("null context");
+            return NAN;
+        }
+        ScfOptions o;
+        o.guess = guess;
+        o.diis = diis;
+        o.conv_e = conv_e;
+        o.P0 = (guess == 2) ? P_inout : nullptr;
+        ScfOut r = scf_run(*ctx->ctx, ctx->one, nelec, imax, conv, o);
+        if (iters) *iters = r.iters;
+        if (P_inout && !r.P.empty()) std::memcpy(P_inout, r.P.data(), sizeof(double) * r.P.size());
+        if (r.status != 0) {
+            set_err(r.msg);
+            return NAN;
+        }
+        return r.E;
+    } catch (std::exception &e) {
+        set_err(e.what());
+        return NAN;
+    }
+}
+
 double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess, int diis, double *P_inout,
                    int *iters)
 {
@@ -604,6 +632,50 @@ double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess
         set_err(e.what());
         return NAN;
     }
+}
+
+static int mlp_status(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return 0;
+    set_err(std::string(what) + ": " + hipGetErrorString(e));
+    return -1;
+}
+
+int librint_mlp_gemm(int M, int N, int K, const uint16_t *A, int lda, int transA, const uint16_t *B, int ldb,
+                     int transB, float *C, int ldc, const float *bias, int act, int accumulate, uint16_t *Cb, int ldcb,
+                     void *stream)
+{
+    return mlp_status(launch_gemm_bf16(M, N, K, A, lda, transA, B, ldb, transB, C, ldc, bias, act, accumulate, Cb, ldcb,
+                                       (hipStream_t)stream),
+                      "librint_mlp_gemm");
+}
+
+int librint_mlp_relu_bwd(size_t n, const float *dz, const float *h, float *dpre, uint16_t *dpre_b, void *stream)
+{
+    return mlp_status(launch_relu_bwd(n, dz, h, dpre, dpre_b, (hipStream_t)stream), "librint_mlp_relu_bwd");
+}
+
+int librint_mlp_colsum(int rows, int cols, const float *x, int ld, float *out, int accumulate, void *stream)
+{
+    return mlp_status(launch_colsum(rows, cols, x, ld, out, accumulate, (hipStream_t)stream), "librint_mlp_colsum");
+}
+
+int librint_mlp_mse(int rows, int cols, const float *y, int ldy, const float *t, const float *mask, int ldt,
+                    float scale, float *dy, uint16_t *dy_b, double *loss, void *stream)
+{
+    return mlp_status(launch_mse(rows, cols, y, ldy, t, mask, ldt, scale, dy, dy_b, loss, (hipStream_t)stream),
+                      "librint_mlp_mse");
+}
+
+int librint_mlp_adam(size_t n, float *w, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                     int step, uint16_t *wb, void *stream)
+{
+    return mlp_status(launch_adam(n, w, g, m, v, lr, b1, b2, eps, step, wb, (hipStream_t)stream), "librint_mlp_adam");
+}
+
+int librint_mlp_to_bf16(size_t n, const float *x, uint16_t *y, void *stream)
+{
+    return mlp_status(launch_f32_to_bf16(n, x, y, (hipStream_t)stream), "librint_mlp_to_bf16");
 }
 
 }  // extern "C"

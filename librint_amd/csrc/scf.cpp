@@ -100,12 +100,21 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
         std::copy(o.P0, o.P0 + nn, P.begin());
     }
     std::vector<std::vector<double>> dF, dE;
-    double delta = 1.0;
+    double delta = 1.0, Eprev = 0.0, dEn = 1.0;
+    const double Enuc = o.conv_e > 0.0 ? nuclear_repulsion(in) : 0.0;
     int it = 0;
-    while (delta > conv && it < imax) {
+    auto converged = [&]() { return delta <= conv && (o.conv_e <= 0.0 || std::fabs(dEn) <= o.conv_e); };
+    while (!converged() && it < imax) {
         Pold = P;
         ctx.fock_host(P.data(), G.data());
         for (size_t i = 0; i < nn; i++) F[i] = H[i] + G[i];
+        if (o.conv_e > 0.0) {
+            // energy of the density entering this iteration
+            double Ek = Enuc;
+            for (size_t i = 0; i < nn; i++) Ek += P[i] * (H[i] + 0.5 * G[i]);
+            dEn = it == 0 ? 1.0 : Ek - Eprev;
+            Eprev = Ek;
+        }
         if (o.diis) {
             // e = X^T (F P S - S P F) X
             std::vector<double> FP(nn), FPS(nn), e(nn), t2(nn);
@@ -148,8 +157,9 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
     }
     out.iters = it;
     out.delta = delta;
+    out.dE = dEn;
     out.P = P;
-    if (delta > conv) {
+    if (!converged()) {
         out.status = 1;
         char buf[160];
         std::snprintf(buf, sizeof buf, "SCF did not converge in %d cycles: |dP| = %.3e > %.3e", imax, delta, conv);

@@ -13,6 +13,7 @@ struct ScfOptions {
     int diis = 0;               // Pulay DIIS on/off
     const double *P0 = nullptr; // starting density for guess == 2
     int trace = 0;
+    double conv_e = 0.0;        // > 0: also require |E_k - E_(k-1)| <= conv_e
 };
 
 struct ScfOut {
@@ -20,6 +21,7 @@ struct ScfOut {
     int iters = 0;
     double E = 0.0;
     double delta = 0.0;
+    double dE = 0.0;  // last energy change (conv_e mode)
     std::vector<double> P;
     std::string msg;
 };
