@@ -255,6 +255,7 @@ class MLP:
 
     def __init__(self, seed=0, device="cuda:0", dims=(NFEAT, HIDDEN, HIDDEN, HIDDEN, NOUT)):
         import torch
+        init_device(device)
         self.torch = torch
         self.dev = torch.device(device)
         self.dims = dims
@@ -373,6 +374,16 @@ class MLP:
 
 # ------------------------------------------------------ data and driver --
 
+def init_device(device="cuda:0"):
+    """Initialise torch's device before the library touches the GPU: torch
+    carries its own HIP runtime, which fails to start once the library's
+    runtime has initialised the device first."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("the learned guess needs a GPU")
+    torch.zeros(1, device=device)
+
+
 def converged_density(m, conv=1e-6, conv_e=1e-10, schwarz=1e-12):
     from .context import FockContext
     ctx = FockContext(m, schwarz)
@@ -385,6 +396,7 @@ def converged_density(m, conv=1e-6, conv_e=1e-10, schwarz=1e-12):
 
 def make_dataset(n_mol, seed, carbons=(2, 3, 4, 5, 6), log=None):
     """Seeded perturbed alkanes with their converged GPU-SCF densities."""
+    init_device()
     rng = np.random.default_rng(seed)
     X, T, W, mols = [], [], [], []
     t0 = time.time()
