@@ -105,11 +105,30 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
     const int knpp = __builtin_amdgcn_readfirstlane(ket.npp);
     const int bnpp = __builtin_amdgcn_readfirstlane(bra.npp);
     const int boff = __builtin_amdgcn_readfirstlane(bra.off);
-    for (int ik = 0; ik < knpp; ik++) {
-        const PrimPair q = B.pp[ket.off + ik];
-        for (int ib = 0; ib < bnpp; ib++) {
-            const PrimPair p = B.pp[boff + ib];
-            prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, rbd, rab, rcd, rt);
+    // Light classes (s on both HRR targets, NF <= 9) are software pipelined:
+    // the next primitive-pair records load while the current primitive
+    // quartet is evaluated, hiding their latency.  Heavier classes keep the
+    // plain loop: the extra registers would cost them occupancy.
+    constexpr bool PIPE = LA == 0 && LC == 0 && C::NF <= 9;
+    if constexpr (PIPE) {
+        PrimPair qn = B.pp[ket.off];
+        for (int ik = 0; ik < knpp; ik++) {
+            const PrimPair q = qn;
+            if (ik + 1 < knpp) qn = B.pp[ket.off + ik + 1];
+            PrimPair pn = B.pp[boff];
+            for (int ib = 0; ib < bnpp; ib++) {
+                const PrimPair p = pn;
+                if (ib + 1 < bnpp) pn = B.pp[boff + ib + 1];
+                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, rbd, rab, rcd, rt);
+            }
+        }
+    } else {
+        for (int ik = 0; ik < knpp; ik++) {
+            const PrimPair q = B.pp[ket.off + ik];
+            for (int ib = 0; ib < bnpp; ib++) {
+                const PrimPair p = B.pp[boff + ib];
+                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, rbd, rab, rcd, rt);
+            }
         }
     }
 }
@@ -352,10 +371,15 @@ __global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict
                                                        const unsigned char *__restrict__ flags,
                                                        const unsigned char *__restrict__ gmask, int accept, DevBasis B,
                                                        const double *__restrict__ D, double *__restrict__ M, int nao,
-                                                       double *__restrict__ jslab, int tfirst)
+                                                       double *__restrict__ jslab, int tfirst,
+                                                       const PrimPair *__restrict__ pp)
 {
     using C = Class<LA, LB, LC, LD>;
     using R = RysLds<LA, LB, LC, LD, T>;
+    // the primitive-pair records through a restrict-qualified argument:
+    // read-only and unaliased, so wavefront-uniform (bra) records can be
+    // fetched with scalar loads
+    B.pp = pp;
     extern __shared__ __align__(16) double lds[];
     T *rt = (T *)lds;                                       // Rys table block (16-byte aligned)
     double *mab_l = lds + RysLdsDoubles<LA, LB, LC, LD, T>(); // bra-bra block
@@ -448,7 +472,7 @@ static hipError_t launch_jk_group_d(const JKGroup *groups, int ngroup, const JKT
     }
     const int g = std::min(grid, ngroup);
     hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, T, DROW>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks,
-                       counter, flags, gmask, accept, B, D, M, nao, jslab, tfirst);
+                       counter, flags, gmask, accept, B, D, M, nao, jslab, tfirst, B.pp);
     return hipGetLastError();
 }
 
