@@ -348,4 +348,85 @@ __device__ __forceinline__ void prim_quartet_fp64(double *eri, const PrimPair &p
     }
 }
 
+// ---- chains in LDS (team kernels: one lane per (root, axis) chain) ----
+
+__device__ __forceinline__ double sel3(int ax, double x, double y, double z) { return ax == 0 ? x : (ax == 1 ? y : z); }
+
+// 2D VRR of one chain in LDS, rolled over m so that the compiler keeps only a
+// few values live (the fully unrolled form forwards every store to its loads
+// and spills).  g(n, m) at n*DJ + m*DL.
+template <int NMAX, int MMAX, int DL, int DJ>
+__device__ __forceinline__ void vrr_rolled(double *g, double cn, double cm, double b10, double b01, double b00)
+{
+    if constexpr (NMAX > 0) {
+        double gm1 = 0.0, g0 = g[0];
+#pragma unroll
+        for (int n = 0; n < NMAX; n++) {
+            const double g1 = cn * g0 + (double)n * b10 * gm1;
+            g[(n + 1) * DJ] = g1;
+            gm1 = g0;
+            g0 = g1;
+        }
+    }
+#pragma unroll 1
+    for (int m = 0; m < MMAX; m++) {
+        const double mb01 = (double)m * b01;
+        double prev = 0.0;  // g(n-1, m)
+#pragma unroll
+        for (int n = 0; n <= NMAX; n++) {
+            const double cur = g[n * DJ + m * DL];
+            double v = cm * cur;
+            if (m > 0) v += mb01 * g[n * DJ + (m - 1) * DL];
+            if (n > 0) v += (double)n * b00 * prev;
+            g[n * DJ + (m + 1) * DL] = v;
+            prev = cur;
+        }
+    }
+}
+
+// HRR of one chain in LDS: b -> a with rab = Rb - Ra (k = 0 plane), then
+// d -> c with rcd = Rd - Rc, rolled over the outer indices.
+template <int LA, int LB, int LC, int NMAX, int MMAX, int DK, int DL, int DJ>
+__device__ __forceinline__ void hrr_rolled(double *g, double rab, double rcd)
+{
+#pragma unroll 1
+    for (int i = 1; i <= LA; i++)
+#pragma unroll 1
+        for (int j = 0; j <= NMAX - i; j++)
+#pragma unroll
+            for (int l = 0; l <= MMAX; l++) {
+                const int p = j * DJ + l * DL + i;
+                g[p] = rab * g[p - 1] + g[p - 1 + DJ];
+            }
+#pragma unroll 1
+    for (int j = 0; j <= LB; j++)
+#pragma unroll 1
+        for (int k = 1; k <= LC; k++)
+#pragma unroll 1
+            for (int l = 0; l <= MMAX - k; l++) {
+                const int p = j * DJ + l * DL + k * DK;
+#pragma unroll
+                for (int n = 0; n < DK; n++) g[p + n] = rcd * g[p + n - DK] + g[p + n - DK + DL];
+            }
+}
+
+// Node and weight of root r only (each chain lane needs one root; evaluating
+// all 2N polynomials per lane would keep 24 coefficient rows live).
+template <int N>
+__device__ __forceinline__ void rys_one(double x, int r, double &v, double &w)
+{
+    if (x >= (double)RYS_XCUT[N]) {
+        const double rs = rsqrt_pos(x);
+        v = RYS_HERM_NODE2[RYS_HOFF[N] + r] * (rs * rs);
+        w = RYS_HERM_W[RYS_HOFF[N] + r] * rs;
+        return;
+    }
+    const int k = (int)x;
+    const double s = 2.0 * (x - (double)k) - 1.0;
+    const double s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
+    const double *c = RYS_CHEB + RYS_OFF[N] + k * (RYS_DEG + 1) * 2 * N;
+    v = rys_estrin11(c + r, 2 * N, s, s2, s4, s8);
+    w = rys_estrin11(c + N + r, 2 * N, s, s2, s4, s8);
+}
+
 } // namespace librint

@@ -502,7 +502,6 @@ __device__ __constant__ signed char CARTP[3][6][3] = {
     {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
     {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
 
-__device__ __forceinline__ double sel3(int ax, double x, double y, double z) { return ax == 0 ? x : (ax == 1 ? y : z); }
 
 template <int LA, int LB, int LC, int LD>
 struct TeamShape {
