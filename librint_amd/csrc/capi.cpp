@@ -11,6 +11,7 @@
 #include "../../include/librint.h"
 #include "context.h"
 #include "int1e.h"
+#include "int1e_gpu.h"
 #include "linalg.h"
 #include "mlp.h"
 #include "scf.h"
@@ -66,7 +67,7 @@ double *int1e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, siz
         }
         Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env);
         double *R = alloc_out((size_t)in.nao * in.nao);
-        int1e_matrix(in, typec, R);
+        int1e_gpu(in, typec == 0 ? R : nullptr, typec == 1 ? R : nullptr, typec == 2 ? R : nullptr);
         return R;
     } catch (std::exception &e) {
         set_err(e.what());
@@ -129,8 +130,7 @@ double energy_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, siz
         }
         Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env);
         std::vector<double> H(n * n), V(n * n);
-        int1e_matrix(in, 1, H.data());
-        int1e_matrix(in, 2, V.data());
+        int1e_gpu(in, nullptr, H.data(), V.data());
         for (size_t i = 0; i < n * n; i++) H[i] += V[i];
         return energy_fast(ctx, in, P, H.data());
     } catch (std::exception &e) {
@@ -206,8 +206,7 @@ GradParts grad_parts(Context *ctx, const Int1eInput &one, const double *P, size_
     if (what & GP_S) {
         // Q = P F P with F = T + V + G[P]
         std::vector<double> F(nn), T(nn), G(nn), PF(nn), Q(nn);
-        int1e_matrix(one, 1, T.data());
-        int1e_matrix(one, 2, F.data());
+        int1e_gpu(one, nullptr, T.data(), F.data());
         ctx->fock_host(P, G.data());
         for (size_t k = 0; k < nn; k++) F[k] += T[k] + G[k];
         gemm_nn(n, P, F.data(), PF.data());

@@ -1,10 +1,11 @@
-// One-electron integrals S, T, V (cartesian) on the host: O(n^2 K^2) SCF
-// input beside the two-electron path.  Semantics of the reference's
-// cint1e_{ovlp,kin,nuc}_cart (src/cint1e.rs:925-1025, src/intor1.rs:168-218,
-// src/g1e.rs:241-395): coefficients as stored in env times common_fac_sp(l),
-// point nuclei, V = sum_C -Z_C <i|1/|r-C||j>, T = <i|-1/2 nabla^2|j>.
-// S and T use the Obara-Saika 1D overlap recursion; V uses Rys quadrature
-// with the same tables as the ERI kernels.
+// One-electron input staging and the host-side derivatives of the
+// one-electron integrals (basis parameters and nuclear coordinates) with the
+// semantics of the reference's cint1e_{ovlp,kin,nuc}_cart
+// (src/cint1e.rs:925-1025, src/intor1.rs:168-218, src/g1e.rs:241-395):
+// coefficients as stored in env times common_fac_sp(l), point nuclei,
+// V = sum_C -Z_C <i|1/|r-C||j>, T = <i|-1/2 nabla^2|j>.  The matrices
+// themselves are computed on the GPU (int1e_kernel.hip).  S and T derivatives use
+// the Obara-Saika 1D overlap recursion, V derivatives Rys quadrature.
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -51,93 +52,6 @@ static void os_1d(int imax, int jmax, double PA, double PB, double p, double s00
                 if (j > 1) v += (j - 1) * h * s[i][j - 2];
             }
             s[i][j] = v;
-        }
-}
-
-void int1e_matrix(const Int1eInput &in, int typ, double *R)
-{
-    const int ns = (int)in.shells.size();
-    const int n = in.nao;
-    std::memset(R, 0, sizeof(double) * (size_t)n * n);
-    for (int ish = 0; ish < ns; ish++)
-        for (int jsh = 0; jsh < ns; jsh++) {
-            const Shell1 &A = in.shells[ish], &B = in.shells[jsh];
-            const int li = A.l, lj = B.l;
-            if (li > 3 || lj > 3) throw std::invalid_argument("l > 3 not supported for one-electron integrals");
-            const int nfi = (li + 1) * (li + 2) / 2, nfj = (lj + 1) * (lj + 2) / 2;
-            int inx[15], iny[15], inz[15], jnx[15], jny[15], jnz[15];
-            cart_pows(li, inx, iny, inz);
-            cart_pows(lj, jnx, jny, jnz);
-            double AB[3], AB2 = 0;
-            for (int d = 0; d < 3; d++) {
-                AB[d] = A.r[d] - B.r[d];
-                AB2 += AB[d] * AB[d];
-            }
-            const double cf = fac_sp(li) * fac_sp(lj);
-            double blk[15 * 15];
-            std::memset(blk, 0, sizeof(blk));
-            for (int jp = 0; jp < B.nprim; jp++)
-                for (int ip = 0; ip < A.nprim; ip++) {
-                    const double a = in.env[A.pexp + ip], b = in.env[B.pexp + jp];
-                    const double ca = in.env[A.pcoef + ip], cb = in.env[B.pcoef + jp];
-                    const double p = a + b;
-                    double P[3];
-                    for (int d = 0; d < 3; d++) P[d] = (a * A.r[d] + b * B.r[d]) / p;
-                    const double pref = ca * cb * cf * std::exp(-a * b / p * AB2);
-                    if (typ <= 1) {
-                        double s[3][8][10];
-                        for (int d = 0; d < 3; d++) os_1d(li, lj + 2, P[d] - A.r[d], P[d] - B.r[d], p, std::sqrt(M_PI / p), s[d]);
-                        for (int jf = 0; jf < nfj; jf++)
-                            for (int f = 0; f < nfi; f++) {
-                                const int ix[3] = {inx[f], iny[f], inz[f]}, jx[3] = {jnx[jf], jny[jf], jnz[jf]};
-                                double val;
-                                if (typ == 0) {
-                                    val = s[0][ix[0]][jx[0]] * s[1][ix[1]][jx[1]] * s[2][ix[2]][jx[2]];
-                                } else {
-                                    double t[3], sv[3];
-                                    for (int d = 0; d < 3; d++) {
-                                        const int i = ix[d], j = jx[d];
-                                        sv[d] = s[d][i][j];
-                                        double tt = -2.0 * b * (2 * j + 1) * s[d][i][j] + 4.0 * b * b * s[d][i][j + 2];
-                                        if (j >= 2) tt += j * (j - 1) * s[d][i][j - 2];
-                                        t[d] = -0.5 * tt;
-                                    }
-                                    val = t[0] * sv[1] * sv[2] + sv[0] * t[1] * sv[2] + sv[0] * sv[1] * t[2];
-                                }
-                                blk[f + nfi * jf] += pref * val;
-                            }
-                    } else {
-                        const int nr = (li + lj) / 2 + 1;
-                        for (size_t c = 0; c < in.charges.size(); c++) {
-                            const double *C = in.centers[c].data();
-                            double PC2 = 0;
-                            for (int d = 0; d < 3; d++) PC2 += (P[d] - C[d]) * (P[d] - C[d]);
-                            double v[9], w[9];
-                            rys_nodes_n(nr, p * PC2, v, w);
-                            const double fac = -in.charges[c] * 2.0 * M_PI / p * pref;
-                            for (int r = 0; r < nr; r++) {
-                                double I[3][8][8];
-                                for (int d = 0; d < 3; d++) {
-                                    const double c00 = (P[d] - A.r[d]) - v[r] * (P[d] - C[d]);
-                                    const double b10 = (1 - v[r]) * 0.5 / p;
-                                    double g[8];
-                                    g[0] = 1;
-                                    if (li + lj > 0) g[1] = c00;
-                                    for (int i = 1; i < li + lj; i++) g[i + 1] = c00 * g[i] + i * b10 * g[i - 1];
-                                    for (int i = 0; i <= li + lj; i++) I[d][i][0] = g[i];
-                                    for (int j = 1; j <= lj; j++)
-                                        for (int i = 0; i <= li + lj - j; i++) I[d][i][j] = I[d][i + 1][j - 1] + AB[d] * I[d][i][j - 1];
-                                }
-                                for (int jf = 0; jf < nfj; jf++)
-                                    for (int f = 0; f < nfi; f++)
-                                        blk[f + nfi * jf] += fac * w[r] * I[0][inx[f]][jnx[jf]] * I[1][iny[f]][jny[jf]] *
-                                                             I[2][inz[f]][jnz[jf]];
-                            }
-                        }
-                    }
-                }
-            for (int jf = 0; jf < nfj; jf++)
-                for (int f = 0; f < nfi; f++) R[(size_t)(A.ao + f) * n + B.ao + jf] = blk[f + nfi * jf];
         }
 }
 

@@ -20,8 +20,6 @@ struct Int1eInput {
 };
 
 Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env);
-// typ 0 overlap, 1 kinetic, 2 nuclear attraction; R[mu*nao + nu]
-void int1e_matrix(const Int1eInput &in, int typ, double *R);
 double nuclear_repulsion(const Int1eInput &in);
 
 // Derivatives of E1 = sum_ab W_ab O_ab (W symmetric, nao*nao; O = S (typ 0),

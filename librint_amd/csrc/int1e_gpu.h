@@ -1,0 +1,28 @@
+// One-electron integrals on the GPU (int1e_kernel.hip) and the host driver that
+// stages a molecule for them.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "int1e.h"
+
+namespace librint {
+
+struct Shell1Dev {
+    int l, nprim, pexp, pcoef, ao, pad;
+    double r[3];
+};
+
+// (i, j) blocks of S, T and/or V (null pointers are skipped) for the ordered
+// shell pairs `pairs` of class (li, lj); nuc[c] = {C, Z_C}.  Matrices nao x nao
+// row-major, device pointers.
+hipError_t launch_int1e(int li, int lj, const int2 *pairs, int npair, const Shell1Dev *sh, const double *env,
+                        const double4 *nuc, int nnuc, int nao, double *S, double *T, double *V, hipStream_t st);
+
+// Host driver: S, T, V (any may be null) into host buffers nao*nao, computed
+// on the current GPU.  Throws on failure; l > 2 is rejected.
+void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V);
+
+}  // namespace librint

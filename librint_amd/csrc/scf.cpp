@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <stdexcept>
 
+#include "int1e_gpu.h"
 #include "linalg.h"
 
 namespace librint {
@@ -63,9 +64,7 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
     if (nelec % 2) throw std::invalid_argument("closed-shell RHF needs an even electron count");
     if (nelec / 2 > n) throw std::invalid_argument("more occupied orbitals than basis functions");
     std::vector<double> S(nn), H(nn), V(nn);
-    int1e_matrix(in, 0, S.data());
-    int1e_matrix(in, 1, H.data());
-    int1e_matrix(in, 2, V.data());
+    int1e_gpu(in, S.data(), H.data(), V.data());
     for (size_t i = 0; i < nn; i++) H[i] += V[i];
 
     std::vector<double> s(n), U(nn), X(nn), tmp(nn), Fp(nn), C(nn), F(nn), G(nn), P(nn), Pold(nn), eps(n);

@@ -1,7 +1,7 @@
 """Host one-electron basis-parameter derivatives (no GPU needed):
 dHcore_c (reference src/dscf.rs:320-336) and dS_u (src/dscf.rs:96-150)
-against central finite differences of int1e_c, and the env split rule
-(src/utils.rs:152-175)."""
+against central finite differences of the oracle's one-electron integrals
+(an independent restatement), and the env split rule (src/utils.rs:152-175)."""
 import numpy as np
 import pytest
 
@@ -25,7 +25,8 @@ def test_split_rule():
 
 @pytest.mark.parametrize("k", range(3))
 def test_dhcore_and_dsu_vs_fd(k):
-    from librint_amd import dscf, scf
+    from librint_amd import dscf
+    from oracle import refcpu
     m = _mols()[k]
     n = m.nao()
     rng = np.random.default_rng(1)
@@ -41,9 +42,9 @@ def test_dhcore_and_dsu_vs_fd(k):
         vals, Ss = [], []
         for sg in (1, -1):
             m._env[j] = env0[j] + sg * h
-            H = scf.int1e(m, 'kin') + scf.int1e(m, 'nuc')
+            H = refcpu.int1e(m, 1) + refcpu.int1e(m, 2)
             vals.append(float((P * H).sum()))
-            Ss.append(scf.int1e(m, 'ovlp'))
+            Ss.append(refcpu.int1e(m, 0))
             m._env[:] = env0
         fd = (vals[0] - vals[1]) / (2 * h)
         fdS = (Ss[0] - Ss[1]) / (2 * h)
@@ -56,7 +57,8 @@ def test_dhcore_and_dsu_vs_fd(k):
 def test_dhcore_planted_mistake_is_caught():
     """A dropped exponent term (the -a_x G(a-1) / +2 shift pair) would show:
     the FD check distinguishes exponent derivatives from half of themselves."""
-    from librint_amd import dscf, scf
+    from librint_amd import dscf
+    from oracle import refcpu
     m = _mols()[1]
     n = m.nao()
     P = np.eye(n)
@@ -66,9 +68,9 @@ def test_dhcore_planted_mistake_is_caught():
     env0 = m._env.copy()
     h = 1e-5 * env0[j]
     m._env[j] = env0[j] + h
-    ep = float((P * (scf.int1e(m, 'kin') + scf.int1e(m, 'nuc'))).sum())
+    ep = float((P * (refcpu.int1e(m, 1) + refcpu.int1e(m, 2))).sum())
     m._env[j] = env0[j] - h
-    em = float((P * (scf.int1e(m, 'kin') + scf.int1e(m, 'nuc'))).sum())
+    em = float((P * (refcpu.int1e(m, 1) + refcpu.int1e(m, 2))).sum())
     m._env[:] = env0
     fd = (ep - em) / (2 * h)
     assert abs(fd) > 1e-3

@@ -102,19 +102,35 @@ def test_header_exports_match_library():
         assert hasattr(_bindings.library, n), n
 
 
+@pytest.mark.gpu
 @pytest.mark.parametrize("name,nelec,n", [("h2", 2, 2), ("h2o", 10, 7)])
 @pytest.mark.parametrize("typ,key", [("ovlp", "S"), ("kin", "T"), ("nuc", "V")])
-def test_library_int1e_truth(name, nelec, n, typ, key):
+def test_library_int1e_truth(gpu, name, nelec, n, typ, key):
+    """int1e_c (GPU kernels) against the reference truth files."""
     from librint_amd import scf
     m = ref_mol(name, "sto3g", nelec)
     assert np.abs(scf.int1e(m, typ) - ref_truth(name, "sto3g_" + key).reshape(n, n)).max() < 1e-12
 
 
+@pytest.mark.gpu
 @pytest.mark.parametrize("typ,key", [("ovlp", "S"), ("kin", "T"), ("nuc", "V")])
-def test_library_int1e_truth_c6h6(typ, key):
+def test_library_int1e_truth_c6h6(gpu, typ, key):
+    """d shells (6-31G* carbon), 102 x 102."""
     from librint_amd import scf
     m = ref_mol("c6h6", "631g", 42)
     assert np.abs(scf.int1e(m, typ) - ref_truth("c6h6", "631g_" + key).reshape(102, 102)).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_library_int1e_vs_oracle_cluster(gpu):
+    """(H2O)2 even-tempered s,p,d against the oracle's restatement."""
+    from librint_amd import scf, mol
+    from oracle import refcpu
+    m = mol.water_cluster(2)
+    for typ, code in (("ovlp", 0), ("kin", 1), ("nuc", 2)):
+        R = scf.int1e(m, typ)
+        O = refcpu.int1e(m, code)
+        assert np.abs(R - O).max() < 1e-11 * max(1.0, np.abs(O).max()), typ
 
 
 def test_library_rejects_bad_codes():
