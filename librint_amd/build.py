@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 SO = os.path.join(LIBDIR, "librint.so")
-SOURCES = ["kernels.hip", "grad.hip", "mlp.hip", "int1e_kernel.hip", "int1e_gpu.cpp", "context.cpp", "int1e.cpp", "linalg.cpp", "scf.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "grad.hip", "mlp.hip", "int1e_kernel.hip", "int1e_gpu.cpp", "context.cpp", "int1e.cpp", "scf.cpp", "capi.cpp"]
 ARCH = os.environ.get("LIBRINT_OFFLOAD_ARCH", "gfx950")
 
 
@@ -47,7 +47,8 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = SO + ".tmp"
-    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp, *objs,
+           "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (r.stdout, r.stderr))

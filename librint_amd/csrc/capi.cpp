@@ -12,7 +12,6 @@
 #include "context.h"
 #include "int1e.h"
 #include "int1e_gpu.h"
-#include "linalg.h"
 #include "mlp.h"
 #include "scf.h"
 
@@ -205,12 +204,11 @@ GradParts grad_parts(Context *ctx, const Int1eInput &one, const double *P, size_
     }
     if (what & GP_S) {
         // Q = P F P with F = T + V + G[P]
-        std::vector<double> F(nn), T(nn), G(nn), PF(nn), Q(nn);
+        std::vector<double> F(nn), T(nn), G(nn), Q(nn);
         int1e_gpu(one, nullptr, T.data(), F.data());
         ctx->fock_host(P, G.data());
         for (size_t k = 0; k < nn; k++) F[k] += T[k] + G[k];
-        gemm_nn(n, P, F.data(), PF.data());
-        gemm_nn(n, PF.data(), P, Q.data());
+        pfp_gpu(n, P, F.data(), Q.data());
         // symmetrise against rounding so that the bra-derivative identity holds
         for (int a = 0; a < n; a++)
             for (int b = 0; b < a; b++) {

@@ -36,11 +36,10 @@ struct DevBuf {
 
 }  // namespace
 
-void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V)
+void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, hipStream_t st)
 {
     const int ns = (int)in.shells.size();
-    const size_t n = (size_t)in.nao;
-    if (!S && !T && !V) return;
+    if (!dS && !dT && !dV) return;
     std::vector<Shell1Dev> sh(ns);
     for (int s = 0; s < ns; s++) {
         const Shell1 &x = in.shells[s];
@@ -66,13 +65,21 @@ void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V)
     DevBuf<double> denv(in.env);
     DevBuf<double4> dnuc(nuc);
     DevBuf<int2> dpairs(pairs);
-    const size_t nn = n * n;
-    DevBuf<double> dS(S ? nn : 0), dT(T ? nn : 0), dV(V ? nn : 0);
     for (int li = 0; li < 3; li++)
         for (int lj = 0; lj < 3; lj++)
             ck(launch_int1e(li, lj, dpairs.p + first[li][lj], count[li][lj], dsh.p, denv.p, dnuc.p, (int)nuc.size(),
-                            (int)n, dS.p, dT.p, dV.p, 0),
+                            in.nao, dS, dT, dV, st),
                "int1e launch");
+    // the staging buffers are freed on return: finish the launches first
+    ck(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V)
+{
+    const size_t nn = (size_t)in.nao * in.nao;
+    if (!S && !T && !V) return;
+    DevBuf<double> dS(S ? nn : 0), dT(T ? nn : 0), dV(V ? nn : 0);
+    int1e_gpu_device(in, dS.p, dT.p, dV.p, 0);
     if (S) ck(hipMemcpy(S, dS.p, sizeof(double) * nn, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     if (T) ck(hipMemcpy(T, dT.p, sizeof(double) * nn, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     if (V) ck(hipMemcpy(V, dV.p, sizeof(double) * nn, hipMemcpyDeviceToHost), "hipMemcpy D2H");

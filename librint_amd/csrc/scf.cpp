@@ -1,24 +1,149 @@
-// RHF driver on top of the GPU Fock build.
+// RHF driver on the GPU: one-electron matrices from the GPU kernels, the
+// fused J/K Fock build, and the dense linear algebra (orthogonaliser,
+// F' = X^T F X, eigenvectors, density, DIIS error vectors) on rocBLAS and
+// rocSOLVER; only the small DIIS system is solved on the host.
 //
 // Reference mode (default, reference src/scf.rs:743-799): P0 = identity,
 // plain Roothaan iteration F -> X^T F X -> eigh -> aufbau P, stop when
 // |P - P_old|_F / 2 <= conv, then the validity checks of src/scf.rs:572-600
 // and the energy E = tr(P(T+V)) + 1/2 tr(P G[P]) + E_nuc (src/scf.rs:827-956).
-// Options: core-Hamiltonian or caller-supplied starting density, and Pulay
-// DIIS extrapolation (not in the reference; off unless requested).
+// Options: core-Hamiltonian or caller-supplied starting density, Pulay DIIS
+// extrapolation and an energy criterion (not in the reference; off unless
+// requested).
+//
+// Storage is column-major (rocBLAS); every matrix that crosses the Fock build
+// or the caller is symmetric, so the layouts agree there.
 #include "scf.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <stdexcept>
+#include <string>
 
 #include "int1e_gpu.h"
-#include "linalg.h"
 
 namespace librint {
 
-static bool solve_linear(int n, std::vector<double> A, std::vector<double> b, std::vector<double> &x)
+namespace {
+
+void hck(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+void bck(rocblas_status s, const char *what)
+{
+    if (s != rocblas_status_success) throw std::runtime_error(std::string(what) + ": rocBLAS status " + std::to_string((int)s));
+}
+
+struct Dev {
+    double *p = nullptr;
+    explicit Dev(size_t n) { hck(hipMalloc(&p, sizeof(double) * std::max<size_t>(n, 1)), "hipMalloc"); }
+    ~Dev()
+    {
+        if (p) (void)hipFree(p);
+    }
+    Dev(const Dev &) = delete;
+    Dev &operator=(const Dev &) = delete;
+};
+
+struct Blas {
+    rocblas_handle h = nullptr;
+    Blas()
+    {
+        bck(rocblas_create_handle(&h), "rocblas_create_handle");
+        bck(rocblas_set_stream(h, 0), "rocblas_set_stream");
+    }
+    ~Blas()
+    {
+        if (h) rocblas_destroy_handle(h);
+    }
+    // C = alpha op(A) op(B) + beta C, all n x n
+    void gemm(int n, bool ta, bool tb, double alpha, const double *A, const double *B, double beta, double *C,
+              int k = -1)
+    {
+        const int kk = k < 0 ? n : k;
+        bck(rocblas_dgemm(h, ta ? rocblas_operation_transpose : rocblas_operation_none,
+                          tb ? rocblas_operation_transpose : rocblas_operation_none, n, n, kk, &alpha, A, n, B, n,
+                          &beta, C, n),
+            "rocblas_dgemm");
+    }
+    double dot(size_t m, const double *x, const double *y)
+    {
+        double r = 0.0;
+        bck(rocblas_ddot(h, (rocblas_int)m, x, 1, y, 1, &r), "rocblas_ddot");
+        return r;
+    }
+    void axpy(size_t m, double a, const double *x, double *y)
+    {
+        bck(rocblas_daxpy(h, (rocblas_int)m, &a, x, 1, y, 1), "rocblas_daxpy");
+    }
+    double nrm2(size_t m, const double *x)
+    {
+        double r = 0.0;
+        bck(rocblas_dnrm2(h, (rocblas_int)m, x, 1, &r), "rocblas_dnrm2");
+        return r;
+    }
+    double amax(size_t m, const double *x)
+    {
+        rocblas_int i = 0;
+        bck(rocblas_idamax(h, (rocblas_int)m, x, 1, &i), "rocblas_idamax");
+        double v = 0.0;
+        if (i >= 1) hck(hipMemcpy(&v, x + (i - 1), sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        return std::fabs(v);
+    }
+    void copy(size_t m, const double *x, double *y)
+    {
+        hck(hipMemcpyAsync(y, x, sizeof(double) * m, hipMemcpyDeviceToDevice, 0), "hipMemcpyAsync");
+    }
+    // eigenvalues ascending into w (device), eigenvectors over A (columns)
+    void eigh(int n, double *A, double *w, double *e, rocblas_int *info)
+    {
+        bck(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, A, n, w, e, info), "rocsolver_dsyevd");
+        rocblas_int hinfo = 0;
+        hck(hipMemcpy(&hinfo, info, sizeof(hinfo), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        if (hinfo != 0) throw std::runtime_error("rocsolver_dsyevd did not converge (info " + std::to_string(hinfo) + ")");
+    }
+};
+
+__global__ void add_kernel(size_t m, const double *__restrict__ a, const double *__restrict__ b, double *__restrict__ c)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) c[i] = a[i] + b[i];
+}
+
+// X[:, k] = U[:, k] / sqrt(s[k])
+__global__ void scale_cols_kernel(int n, const double *__restrict__ U, const double *__restrict__ s,
+                                  double *__restrict__ X)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * n) return;
+    const int k = (int)(i / n);
+    X[i] = U[i] * rsqrt(s[k]);
+}
+
+// e = A - A^T
+__global__ void antisym_kernel(int n, const double *__restrict__ A, double *__restrict__ e)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * n) return;
+    const int r = (int)(i % n), c = (int)(i / n);
+    e[i] = A[i] - A[(size_t)r * n + c];
+}
+
+__global__ void identity_kernel(int n, double *__restrict__ A)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * n) return;
+    A[i] = (i % n == i / n) ? 1.0 : 0.0;
+}
+
+unsigned blocks(size_t m) { return (unsigned)((m + 255) / 256); }
+
+bool solve_linear(int n, std::vector<double> A, std::vector<double> b, std::vector<double> &x)
 {
     for (int c = 0; c < n; c++) {
         int p = c;
@@ -44,6 +169,20 @@ static bool solve_linear(int n, std::vector<double> A, std::vector<double> b, st
     return true;
 }
 
+}  // namespace
+
+void pfp_gpu(int n, const double *P, const double *F, double *Q)
+{
+    const size_t nn = (size_t)n * n;
+    Blas B;
+    Dev dP(nn), dF(nn), dT(nn), dQ(nn);
+    hck(hipMemcpy(dP.p, P, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    hck(hipMemcpy(dF.p, F, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    B.gemm(n, false, false, 1.0, dP.p, dF.p, 0.0, dT.p);
+    B.gemm(n, false, false, 1.0, dT.p, dP.p, 0.0, dQ.p);
+    hck(hipMemcpy(Q, dQ.p, sizeof(double) * nn, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
 double energy_fast(Context &ctx, const Int1eInput &in, const double *P, const double *H)
 {
     const int n = ctx.nao();
@@ -63,101 +202,128 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
     if (in.nao != n) throw std::runtime_error("basis mismatch between 1e and 2e inputs");
     if (nelec % 2) throw std::invalid_argument("closed-shell RHF needs an even electron count");
     if (nelec / 2 > n) throw std::invalid_argument("more occupied orbitals than basis functions");
-    std::vector<double> S(nn), H(nn), V(nn);
-    int1e_gpu(in, S.data(), H.data(), V.data());
-    for (size_t i = 0; i < nn; i++) H[i] += V[i];
+    const int nocc = nelec / 2;
+    Blas B;
+    Dev S(nn), H(nn), V(nn), X(nn), T1(nn), T2(nn), F(nn), G(nn), P(nn), Pold(nn), w(n), e(n);
+    rocblas_int *info = nullptr;
+    hck(hipMalloc(&info, sizeof(rocblas_int)), "hipMalloc");
+    struct InfoFree {
+        rocblas_int *p;
+        ~InfoFree() { (void)hipFree(p); }
+    } info_free{info};
+    int1e_gpu_device(in, S.p, H.p, V.p, 0);
+    B.axpy(nn, 1.0, V.p, H.p);  // H = T + V
 
-    std::vector<double> s(n), U(nn), X(nn), tmp(nn), Fp(nn), C(nn), F(nn), G(nn), P(nn), Pold(nn), eps(n);
-    eigh(n, S.data(), s.data(), U.data());
-    if (s[0] < 1e-8) {
+    // X = U s^{-1/2}
+    B.copy(nn, S.p, T1.p);
+    B.eigh(n, T1.p, w.p, e.p, info);
+    double smin = 0.0;
+    hck(hipMemcpy(&smin, w.p, sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    if (smin < 1e-8) {
         out.status = 3;
         out.msg = "overlap matrix is near-singular (smallest eigenvalue < 1e-8)";
         return out;
     }
-    for (int i = 0; i < n; i++)
-        for (int k = 0; k < n; k++) X[(size_t)i * n + k] = U[(size_t)i * n + k] / std::sqrt(s[k]);
-    const int nocc = nelec / 2;
-    auto density_from_F = [&](const std::vector<double> &Fm) {
-        gemm_tn(n, X.data(), Fm.data(), tmp.data());  // X^T F
-        gemm_nn(n, tmp.data(), X.data(), Fp.data());  // X^T F X
-        eigh(n, Fp.data(), eps.data(), tmp.data());
-        gemm_nn(n, X.data(), tmp.data(), C.data());
-        for (int mu = 0; mu < n; mu++)
-            for (int nu = 0; nu < n; nu++) {
-                double acc = 0;
-                for (int i = 0; i < nocc; i++) acc += C[(size_t)mu * n + i] * C[(size_t)nu * n + i];
-                P[(size_t)mu * n + nu] = 2.0 * acc;
-            }
+    hipLaunchKernelGGL(scale_cols_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, T1.p, w.p, X.p);
+    hck(hipGetLastError(), "scale_cols launch");
+
+    // P from F: F' = X^T F X, eigenvectors C', C = X C', P = 2 C_occ C_occ^T
+    auto density_from_F = [&](const double *Fm) {
+        B.gemm(n, false, false, 1.0, Fm, X.p, 0.0, T1.p);   // F X
+        B.gemm(n, true, false, 1.0, X.p, T1.p, 0.0, T2.p);  // X^T F X
+        B.eigh(n, T2.p, w.p, e.p, info);
+        B.gemm(n, false, false, 1.0, X.p, T2.p, 0.0, T1.p);  // C
+        B.gemm(n, false, true, 2.0, T1.p, T1.p, 0.0, P.p, nocc);
     };
     if (o.guess == 0) {
-        std::fill(P.begin(), P.end(), 0.0);
-        for (int i = 0; i < n; i++) P[(size_t)i * n + i] = 1.0;
+        hipLaunchKernelGGL(identity_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, P.p);
+        hck(hipGetLastError(), "identity launch");
     } else if (o.guess == 1) {
-        density_from_F(H);
+        density_from_F(H.p);
     } else {
         if (!o.P0) throw std::invalid_argument("guess=2 needs a starting density");
-        std::copy(o.P0, o.P0 + nn, P.begin());
+        hck(hipMemcpy(P.p, o.P0, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
     }
-    std::vector<std::vector<double>> dF, dE;
+    const int NDIIS = 8;
+    std::vector<Dev *> dF, dE;
+    struct Hist {
+        std::vector<Dev *> &a, &b;
+        ~Hist()
+        {
+            for (Dev *d : a) delete d;
+            for (Dev *d : b) delete d;
+        }
+    } hist{dF, dE};
+    std::vector<double> Bm;  // DIIS overlaps, row-major NDIIS x NDIIS of the current history
     double delta = 1.0, Eprev = 0.0, dEn = 1.0;
-    const double Enuc = o.conv_e > 0.0 ? nuclear_repulsion(in) : 0.0;
+    const double Enuc = nuclear_repulsion(in);
     int it = 0;
     auto converged = [&]() { return delta <= conv && (o.conv_e <= 0.0 || std::fabs(dEn) <= o.conv_e); };
     while (!converged() && it < imax) {
-        Pold = P;
-        ctx.fock_host(P.data(), G.data());
-        for (size_t i = 0; i < nn; i++) F[i] = H[i] + G[i];
+        B.copy(nn, P.p, Pold.p);
+        ctx.fock_device(P.p, G.p, 0);
+        hipLaunchKernelGGL(add_kernel, dim3(blocks(nn)), dim3(256), 0, 0, nn, H.p, G.p, F.p);
+        hck(hipGetLastError(), "add launch");
         if (o.conv_e > 0.0) {
-            // energy of the density entering this iteration
-            double Ek = Enuc;
-            for (size_t i = 0; i < nn; i++) Ek += P[i] * (H[i] + 0.5 * G[i]);
+            const double Ek = Enuc + B.dot(nn, P.p, H.p) + 0.5 * B.dot(nn, P.p, G.p);
             dEn = it == 0 ? 1.0 : Ek - Eprev;
             Eprev = Ek;
         }
         if (o.diis) {
             // e = X^T (F P S - S P F) X
-            std::vector<double> FP(nn), FPS(nn), e(nn), t2(nn);
-            gemm_nn(n, F.data(), P.data(), FP.data());
-            gemm_nn(n, FP.data(), S.data(), FPS.data());
-            for (int i = 0; i < n; i++)
-                for (int j = 0; j < n; j++) e[(size_t)i * n + j] = FPS[(size_t)i * n + j] - FPS[(size_t)j * n + i];
-            gemm_tn(n, X.data(), e.data(), t2.data());
-            gemm_nn(n, t2.data(), X.data(), e.data());
-            dF.push_back(F);
-            dE.push_back(e);
-            if ((int)dF.size() > 8) {
+            B.gemm(n, false, false, 1.0, F.p, P.p, 0.0, T1.p);
+            B.gemm(n, false, false, 1.0, T1.p, S.p, 0.0, T2.p);
+            hipLaunchKernelGGL(antisym_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, T2.p, T1.p);
+            hck(hipGetLastError(), "antisym launch");
+            B.gemm(n, true, false, 1.0, X.p, T1.p, 0.0, T2.p);
+            Dev *en = nullptr, *fn = nullptr;
+            if ((int)dF.size() == NDIIS) {  // recycle the oldest buffers
+                fn = dF.front();
+                en = dE.front();
                 dF.erase(dF.begin());
                 dE.erase(dE.begin());
+                const int m0 = NDIIS;
+                std::vector<double> nb((NDIIS) * (NDIIS), 0.0);
+                for (int i = 1; i < m0; i++)
+                    for (int j = 1; j < m0; j++) nb[(i - 1) * NDIIS + (j - 1)] = Bm[i * NDIIS + j];
+                Bm.swap(nb);
+            } else {
+                fn = new Dev(nn);
+                en = new Dev(nn);
             }
+            B.gemm(n, false, false, 1.0, T2.p, X.p, 0.0, en->p);  // X^T (FPS - SPF) X
+            B.copy(nn, F.p, fn->p);
+            dF.push_back(fn);
+            dE.push_back(en);
             const int m = (int)dF.size();
+            if ((int)Bm.size() != NDIIS * NDIIS) Bm.assign(NDIIS * NDIIS, 0.0);
+            for (int j = 0; j < m; j++) {
+                const double d = B.dot(nn, dE[m - 1]->p, dE[j]->p);
+                Bm[(m - 1) * NDIIS + j] = Bm[j * NDIIS + (m - 1)] = d;
+            }
             if (m >= 2) {
-                std::vector<double> B((m + 1) * (m + 1), 0.0), rhs(m + 1, 0.0), c;
+                std::vector<double> A((m + 1) * (m + 1), 0.0), rhs(m + 1, 0.0), c;
                 for (int i = 0; i < m; i++)
-                    for (int j = 0; j <= i; j++) {
-                        double d = 0;
-                        for (size_t k = 0; k < nn; k++) d += dE[i][k] * dE[j][k];
-                        B[i * (m + 1) + j] = B[j * (m + 1) + i] = d;
-                    }
-                for (int i = 0; i < m; i++) B[i * (m + 1) + m] = B[m * (m + 1) + i] = -1.0;
+                    for (int j = 0; j < m; j++) A[i * (m + 1) + j] = Bm[i * NDIIS + j];
+                for (int i = 0; i < m; i++) A[i * (m + 1) + m] = A[m * (m + 1) + i] = -1.0;
                 rhs[m] = -1.0;
-                if (solve_linear(m + 1, B, rhs, c)) {
-                    std::fill(F.begin(), F.end(), 0.0);
-                    for (int i = 0; i < m; i++)
-                        for (size_t k = 0; k < nn; k++) F[k] += c[i] * dF[i][k];
+                if (solve_linear(m + 1, A, rhs, c)) {
+                    hck(hipMemsetAsync(F.p, 0, sizeof(double) * nn, 0), "hipMemsetAsync");
+                    for (int i = 0; i < m; i++) B.axpy(nn, c[i], dF[i]->p, F.p);
                 }
             }
         }
-        density_from_F(F);
-        delta = 0;
-        for (size_t i = 0; i < nn; i++) delta += (P[i] - Pold[i]) * (P[i] - Pold[i]);
-        delta = std::sqrt(delta) / 2.0;
+        density_from_F(F.p);
+        B.axpy(nn, -1.0, P.p, Pold.p);
+        delta = B.nrm2(nn, Pold.p) / 2.0;
         it++;
         if (o.trace) std::fprintf(stderr, "librint scf iter %d delta %.3e\n", it, delta);
     }
     out.iters = it;
     out.delta = delta;
     out.dE = dEn;
-    out.P = P;
+    out.P.resize(nn);
+    hck(hipMemcpy(out.P.data(), P.p, sizeof(double) * nn, hipMemcpyDeviceToHost), "hipMemcpy D2H");
     if (!converged()) {
         out.status = 1;
         char buf[160];
@@ -166,24 +332,22 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
         return out;
     }
     // validity checks (reference src/scf.rs:572-600)
-    double tr = 0;
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) tr += P[(size_t)i * n + j] * S[(size_t)j * n + i];
+    const double tr = B.dot(nn, P.p, S.p);
     if (std::fabs(tr - nelec) > 1e-6) {
         out.status = 2;
         out.msg = "converged density has tr(PS) != nelec";
         return out;
     }
-    gemm_nn(n, P.data(), S.data(), tmp.data());
-    gemm_nn(n, tmp.data(), P.data(), Fp.data());
-    double idem = 0;
-    for (size_t i = 0; i < nn; i++) idem = std::max(idem, std::fabs(Fp[i] - 2.0 * P[i]));
-    if (idem > 1e-6) {
+    B.gemm(n, false, false, 1.0, P.p, S.p, 0.0, T1.p);
+    B.gemm(n, false, false, 1.0, T1.p, P.p, 0.0, T2.p);
+    B.axpy(nn, -2.0, P.p, T2.p);
+    if (B.amax(nn, T2.p) > 1e-6) {
         out.status = 2;
         out.msg = "converged density is not idempotent";
         return out;
     }
-    out.E = energy_fast(ctx, in, P.data(), H.data());
+    ctx.fock_device(P.p, G.p, 0);
+    out.E = Enuc + B.dot(nn, P.p, H.p) + 0.5 * B.dot(nn, P.p, G.p);
     out.status = 0;
     return out;
 }

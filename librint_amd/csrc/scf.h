@@ -27,6 +27,8 @@ struct ScfOut {
 };
 
 ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double conv, const ScfOptions &o);
+// Q = P F P for symmetric host matrices (row- and column-major agree), on the GPU
+void pfp_gpu(int n, const double *P, const double *F, double *Q);
 // E[P] = tr(P H) + 1/2 tr(P G[P]) + E_nuc with G from the GPU Fock build
 double energy_fast(Context &ctx, const Int1eInput &in, const double *P, const double *H);
 
