@@ -116,6 +116,7 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
             const PrimPair q = qn;
             if (ik + 1 < knpp) qn = B.pp[ket.off + ik + 1];
             PrimPair pn = B.pp[boff];
+#pragma unroll 2
             for (int ib = 0; ib < bnpp; ib++) {
                 const PrimPair p = pn;
                 if (ib + 1 < bnpp) pn = B.pp[boff + ib + 1];
