@@ -32,13 +32,15 @@ extern "C" {
 /* reference src/p2c.rs:67-75 */
 void free_c(double *ptr, size_t len);
 
-/* nao*nao one-electron matrix, R[mu*nao+nu]; coord 0 cartesian (1 spherical:
- * not supported, returns NULL); typec 0 overlap, 1 kinetic, 2 nuclear.
+/* n*n one-electron matrix, R[mu*n+nu]; coord 0 cartesian (n = nao), 1 real
+ * spherical (n = sum of 2l+1, transform of src/cart2sph.rs:88380-88530);
+ * typec 0 overlap, 1 kinetic, 2 nuclear.  Computed on the GPU, l <= 2.
  * reference src/p2c.rs:85-100, src/scf.rs:160-213 */
 double *int1e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, int coord, int typec);
 
-/* nao^4 ERI tensor R[((i*n+j)*n+k)*n+l] = (ij|kl), computed on the GPU.
- * reference src/p2c.rs:108-123, src/scf.rs:215-311 */
+/* n^4 ERI tensor R[((i*n+j)*n+k)*n+l] = (ij|kl), computed on the GPU; coord 0
+ * cartesian (n = nao), 1 real spherical.  reference src/p2c.rs:108-123,
+ * src/scf.rs:215-311 */
 double *int2e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, int coord);
 
 /* Converged RHF density (nao*nao) or NULL; reference SCF protocol (P0 = I,

@@ -11,6 +11,7 @@
 #include "../../include/librint.h"
 #include "context.h"
 #include "int1e.h"
+#include "cart2sph.h"
 #include "int1e_gpu.h"
 #include "mlp.h"
 #include "scf.h"
@@ -39,10 +40,43 @@ static double *alloc_out(size_t n)
 
 static bool coord_ok(int coord)
 {
-    if (coord == 0) return true;
-    if (coord == 1) set_err("spherical (coord=1) output is not provided by the GPU path yet");
-    else set_err("invalid coord code " + std::to_string(coord));
+    if (coord == 0 || coord == 1) return true;
+    set_err("invalid coord code " + std::to_string(coord));
     return false;
+}
+
+// device buffer released on scope exit
+struct DevMem {
+    double *p = nullptr;
+    explicit DevMem(size_t n) { check(hipMalloc(&p, sizeof(double) * std::max<size_t>(n, 1)), "hipMalloc"); }
+    ~DevMem()
+    {
+        if (p) (void)hipFree(p);
+    }
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+};
+
+// spherical form of a cartesian ndim-index array on the device, copied to host
+static double *to_spherical(int ndim, const int *bas, size_t bas_l, const double *dcart, int ncart)
+{
+    int nc = 0;
+    std::vector<C2SRow> rows = c2s_rows(bas, bas_l, &nc);
+    if (nc != ncart) throw std::runtime_error("cartesian count mismatch in the spherical transform");
+    const size_t ns = rows.size();
+    const size_t m = ndim == 2 ? ns * ns : ns * ns * ns * ns;
+    C2SRow *drows = nullptr;
+    check(hipMalloc(&drows, sizeof(C2SRow) * std::max<size_t>(ns, 1)), "hipMalloc");
+    struct RowsFree {
+        C2SRow *p;
+        ~RowsFree() { (void)hipFree(p); }
+    } rf{drows};
+    check(hipMemcpy(drows, rows.data(), sizeof(C2SRow) * ns, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    DevMem dsph(m);
+    check(launch_c2s(ndim, drows, (int)ns, dcart, ncart, dsph.p, 0), "cart2sph launch");
+    double *out = alloc_out(m);
+    check(hipMemcpy(out, dsph.p, sizeof(double) * m, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return out;
 }
 
 extern "C" {
@@ -65,7 +99,13 @@ double *int1e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, siz
             return nullptr;
         }
         Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env);
-        double *R = alloc_out((size_t)in.nao * in.nao);
+        const size_t nn = (size_t)in.nao * in.nao;
+        if (coord == 1) {
+            DevMem d(nn);
+            int1e_gpu_device(in, typec == 0 ? d.p : nullptr, typec == 1 ? d.p : nullptr, typec == 2 ? d.p : nullptr, 0);
+            return to_spherical(2, bas, bas_l, d.p, in.nao);
+        }
+        double *R = alloc_out(nn);
         int1e_gpu(in, typec == 0 ? R : nullptr, typec == 1 ? R : nullptr, typec == 2 ? R : nullptr);
         return R;
     } catch (std::exception &e) {
@@ -80,6 +120,11 @@ double *int2e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, siz
         if (!coord_ok(coord)) return nullptr;
         Context ctx(atm, atm_l, bas, bas_l, env, env_l);
         const size_t n = (size_t)ctx.nao();
+        if (coord == 1) {
+            DevMem d(n * n * n * n);
+            ctx.tensor_device(d.p);
+            return to_spherical(4, bas, bas_l, d.p, (int)n);
+        }
         double *R = alloc_out(n * n * n * n);
         ctx.tensor_host(R);
         return R;

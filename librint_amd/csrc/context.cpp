@@ -664,6 +664,20 @@ void Context::tensor_host(double *R)
     const size_t n4 = n * n * n * n;
     double *dR = nullptr;
     check(hipMalloc(&dR, sizeof(double) * n4), "hipMalloc");
+    try {
+        tensor_device(dR);
+    } catch (...) {
+        dev_free(dR);
+        throw;
+    }
+    check(hipMemcpy(R, dR, sizeof(double) * n4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    dev_free(dR);
+}
+
+void Context::tensor_device(double *dR)
+{
+    const size_t n = (size_t)nao_;
+    const size_t n4 = n * n * n * n;
     check(hipMemset(dR, 0, sizeof(double) * n4), "hipMemset");
     // every canonical pair of pairs, no screening (exact tensor)
     const int np = (int)pairs_.size();
@@ -684,9 +698,8 @@ void Context::tensor_host(double *R)
         bufs.push_back(dt);
         check(launch_list(la, lb, lc, ld, dt, (int)kv.second.size(), B, dR, 2, nao_, 0), "tensor launch");
     }
-    check(hipMemcpy(R, dR, sizeof(double) * n4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     for (auto p : bufs) dev_free(p);
-    dev_free(dR);
 }
 
 size_t Context::shells_host(const int *quartets, int nq, double *out)

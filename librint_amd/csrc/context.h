@@ -75,6 +75,8 @@ public:
 
     // Full nao^4 tensor (host buffer, R[((i*n+j)*n+k)*n+l]).
     void tensor_host(double *R);
+    // Same into a device buffer of nao^4 doubles.
+    void tensor_device(double *dR);
     // Blocks for a list of shell quartets, each in cint2e_cart layout
     // (i fastest), concatenated; returns the number of doubles written.
     size_t shells_host(const int *quartets, int nq, double *out);

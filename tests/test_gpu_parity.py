@@ -160,8 +160,11 @@ def test_failures_are_reported(gpu):
         scf.density(m, 1, 1e-12)  # cannot converge in one cycle
     with pytest.raises(RuntimeError):
         scf.scf(m, 1, 1e-12)
+    from librint_amd import mol
+    atoms = [(8, (0.0, 0.0, 0.0)), (1, (0.0, 1.4, 1.1)), (1, (0.0, -1.4, 1.1))]
+    f_shell = mol.build(atoms, {8: mol.STO3G[8] + [(3, [0.8], [1.0])], 1: mol.STO3G[1]}, "h2o_f")
     with pytest.raises(RuntimeError):
-        scf.int2e(m, "sph")
+        scf.int2e(f_shell)  # l = 3 is outside the GPU kernels
 
 
 def test_fock_device_buffers(gpu):
@@ -277,3 +280,55 @@ def test_mixed_precision_d_shells(gpu):
     Gm = ctx.fock(P)
     assert ctx.flag_counts()["quartets"][1] > 0
     assert abs(0.5 * np.sum(P * (Gm - G64))) < 1e-6
+
+
+def _c2s_independent():
+    """cart -> real spherical rows for l = 2 fitted from scipy's spherical
+    harmonics (r^2 Y_2m on random points, least squares over the six
+    monomials), sign of each row fixed by its largest coefficient."""
+    import scipy.special as sp
+    rng = np.random.default_rng(4)
+    v = rng.normal(size=(400, 3))
+    x, y, z = v.T
+    r = np.linalg.norm(v, axis=1)
+    th, ph = np.arccos(z / r), np.arctan2(y, x)
+    Y = lambda m: (sp.sph_harm_y(2, m, th, ph) if hasattr(sp, "sph_harm_y") else sp.sph_harm(m, 2, ph, th))
+    real = {0: Y(0).real}
+    for m in (1, 2):
+        real[m] = np.sqrt(2) * (-1) ** m * Y(m).real
+        real[-m] = np.sqrt(2) * (-1) ** m * Y(m).imag
+    mono = np.stack([x * x, x * y, x * z, y * y, y * z, z * z], axis=1)
+    rows = []
+    for m in (-2, -1, 0, 1, 2):
+        c, *_ = np.linalg.lstsq(mono, real[m] * r * r, rcond=None)
+        c[np.abs(c) < 1e-12] = 0.0
+        rows.append(c * np.sign(c[np.argmax(np.abs(c))]))
+    return np.array(rows)
+
+
+def test_spherical_output(gpu):
+    """coord = 1: identical to cartesian for s,p bases; identity on-site
+    overlap for a normalised d shell; int2e equal to the cartesian tensor
+    transformed with independently fitted solid harmonics."""
+    from librint_amd import mol, scf
+    m = ref_mol("h2o", "sto3g", 10)
+    for typ in ("ovlp", "kin", "nuc"):
+        np.testing.assert_array_equal(scf.int1e(m, typ, "sph"), scf.int1e(m, typ, "cart"))
+    np.testing.assert_array_equal(scf.int2e(m, "sph"), scf.int2e(m, "cart"))
+    md = mol.h2o_631gstar_like()
+    Ssph = scf.int1e(md, "ovlp", "sph")
+    # O shells: 1s, 2s, 2p, d -> the d block is spherical functions 5..9
+    np.testing.assert_allclose(Ssph[5:10, 5:10], np.eye(5), atol=1e-12)
+    # transform matrix: identity for s/p, fitted rows for the d shell
+    ncart, nsph = md.nao(0), md.nao(1)
+    C = np.zeros((nsph, ncart))
+    C[:5, :5] = np.eye(5)
+    C[5:10, 5:11] = _c2s_independent()
+    C[10:, 11:] = np.eye(nsph - 10)
+    for typ in ("ovlp", "kin", "nuc"):
+        np.testing.assert_allclose(scf.int1e(md, typ, "sph"), C @ scf.int1e(md, typ, "cart") @ C.T, atol=1e-12)
+    Rc = scf.int2e(md, "cart")
+    Rs = scf.int2e(md, "sph")
+    ref = np.einsum("ai,bj,ck,dl,ijkl->abcd", C, C, C, C, Rc, optimize=True)
+    assert Rs.shape == (nsph,) * 4
+    assert np.abs(Rs - ref).max() < 1e-12
