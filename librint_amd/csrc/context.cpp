@@ -125,6 +125,7 @@ Context::~Context()
     dev_free(d_pp_idx_);
     dev_free(d_shell_slot_);
     dev_free(d_slot_coef_);
+    dev_free(d_gtasks1_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
@@ -284,6 +285,7 @@ void Context::set_schwarz(double tau)
 {
     tau_ = tau;
     build_tasks();
+    gtasks_valid_ = false;
 }
 
 // Per primitive quartet flop count of the fp64 kernel's algorithm (the
@@ -820,6 +822,49 @@ size_t Context::shells_host(const int *quartets, int nq, double *out)
     return (size_t)out_total;
 }
 
+void Context::build_grad_tasks()
+{
+    // Ket-major list of the quartets of build_tasks (bins X >= Y, Q_i Q_j >
+    // tau, j <= i inside one bin): for each ket pair j of bin Y the bras of
+    // bin X form a prefix in Q order, cut into windows of 64.
+    glaunch1_.clear();
+    dev_free(d_gtasks1_);
+    const int nb = (int)bins_.size();
+    std::map<std::pair<int, int>, std::vector<JKTask>> acc;  // (bra class, ket class)
+    for (int X = 0; X < nb; X++) {
+        const Bin &bx = bins_[X];
+        for (int Y = 0; Y <= X; Y++) {
+            const Bin &by = bins_[Y];
+            std::vector<JKTask> &v = acc[std::make_pair(bx.cls, by.cls)];
+            for (int j = 0; j < by.count; j++) {
+                const int k = by.first + j;
+                const double Qk = pairs_[k].Q;
+                int lo = 0, hi = bx.count;
+                while (lo < hi) {
+                    const int mid = (lo + hi) / 2;
+                    if (pairs_[bx.first + mid].Q * Qk > tau_) lo = mid + 1;
+                    else hi = mid;
+                }
+                for (int i0 = X == Y ? j : 0; i0 < lo; i0 += 64)
+                    v.push_back(JKTask{k, bx.first + i0, std::min(64, lo - i0), 0});
+            }
+        }
+    }
+    std::vector<JKTask> all;
+    for (auto &kv : acc) {
+        if (kv.second.empty()) continue;
+        GradLaunch G;
+        cls_ls(kv.first.first, G.la, G.lb);
+        cls_ls(kv.first.second, G.lc, G.ld);
+        G.first = (int)all.size();
+        G.count = (int)kv.second.size();
+        all.insert(all.end(), kv.second.begin(), kv.second.end());
+        glaunch1_.push_back(G);
+    }
+    if (!all.empty()) d_gtasks1_ = dev_upload(all);
+    gtasks_valid_ = true;
+}
+
 void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
 {
     // per-shell primitive slots: exponent and coefficient derivatives are
@@ -843,6 +888,7 @@ void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
         d_shell_slot_ = dev_upload(info);
         d_slot_coef_ = dev_upload(cf);
     }
+    if (!gtasks_valid_) build_grad_tasks();
     const size_t nn = (size_t)nao_ * nao_;
     const int na = natm();
     if (!d_D_) check(hipMalloc(&d_D_, sizeof(double) * nn), "hipMalloc");
@@ -854,10 +900,13 @@ void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
     const DevBasis B = devb();
     check(hipEventRecord(ev0_, 0), "hipEventRecord");
     for (const ClassLaunch &L : launches_)
-        for (int side = 0; side < 2; side++)
-            check(launch_grad(L.la, L.lb, L.lc, L.ld, side, d_tasks_ + L.first, L.count, B, d_pp_idx_, d_shell_slot_,
-                              d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, 0),
-                  "grad launch");
+        check(launch_grad(L.la, L.lb, L.lc, L.ld, 0, d_tasks_ + L.first, L.count, B, d_pp_idx_, d_shell_slot_,
+                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, 0),
+              "grad launch");
+    for (const GradLaunch &G : glaunch1_)
+        check(launch_grad(G.la, G.lb, G.lc, G.ld, 1, d_gtasks1_ + G.first, G.count, B, d_pp_idx_, d_shell_slot_,
+                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, 0),
+              "grad launch");
     check(hipEventRecord(ev1_, 0), "hipEventRecord");
     std::vector<double> h(nout);
     check(hipMemcpy(h.data(), dg, sizeof(double) * nout, hipMemcpyDeviceToHost), "hipMemcpy D2H");

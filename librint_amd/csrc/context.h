@@ -162,6 +162,15 @@ private:
     int2 *d_shell_slot_ = nullptr;  // {atom, first primitive slot} per shell
     double *d_slot_coef_ = nullptr; // contraction coefficient per slot
     int nslot_ = 0;
+    // adjoint kernel, ket side: the same quartets listed ket-major (the task's
+    // fixed pair is the canonical ket, its window runs over canonical bras)
+    struct GradLaunch {
+        int la, lb, lc, ld, first, count;
+    };
+    std::vector<GradLaunch> glaunch1_;
+    JKTask *d_gtasks1_ = nullptr;
+    bool gtasks_valid_ = false;
+    void build_grad_tasks();
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<hipEvent_t> evc_;  // one event after each class launch
     DevBasis devb() const;

@@ -8,6 +8,10 @@ namespace librint {
 
 // Adjoint two-electron kernel: one launch per angular class and side
 // (0: bra centres a, b; 1: ket centres c, d) over the class's task list.
+// Each task's fixed pair (JKTask::bra) is the differentiated pair: side 0
+// takes the bra-major list of the Fock build (fixed canonical bra, window of
+// canonical kets), side 1 a ket-major list of the same quartets (fixed
+// canonical ket, window of up to 64 canonical bras of one bin).
 //   gatom[3*natm]   += dE2/dR (nuclear coordinates)
 //   dprim[2*slot]   += dE2/d(exponent), dprim[2*slot+1] += dE2/d(coefficient)
 // where slot = shell_info[s].y + primitive index (one slot per primitive of

@@ -20,10 +20,11 @@
 // Work mapping: a team of TS lanes owns one quartet; lanes c < 3*NROOTS
 // build the (root, axis) chains of the raised recurrence in LDS, then every
 // lane contracts its share of the weighted outputs.  The 64/TS teams of a
-// wavefront walk the kets of one task (one bin: same class and primitive
-// counts, so all loops are uniform).  Derivatives go to per-shell-primitive
-// slots (not to env entries shared by every atom of an element) to keep the
-// atomics spread over many addresses.
+// wavefront walk the window pairs of one task (one bin: same class and
+// primitive counts, so all loops are uniform) while the task's fixed pair is
+// the differentiated one.  Derivatives go to per-shell-primitive slots (not
+// to env entries shared by every atom of an element); each task adds its
+// wavefront-reduced sums with one atomic per slot and value.
 #include <hip/hip_runtime.h>
 
 #include "eri_core.h"
@@ -157,18 +158,30 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
     const int lane = threadIdx.x;
     const int team = lane / S::TS, tl = lane - team * S::TS;
     double *gt = g + team * S::NCH * S::GS;
-    const GPair bra = gload_pair(B, t.bra);
+    // The task's fixed pair is the differentiated one (SIDE 0: the canonical
+    // bra; SIDE 1: the canonical ket, from a ket-major task list whose window
+    // runs over canonical bras), so every team of the wavefront adds to the
+    // same derivative slots and the sums are reduced in registers over the
+    // whole task before one set of atomics.
+    const GPair sd = gload_pair(B, t.bra);
+    const int2 si0 = shell_info[sd.sa], si1 = shell_info[sd.sb];
     const size_t n = (size_t)nao;
     const int niter = (t.nket + S::NT - 1) / S::NT;
+    // nuclear derivatives of the two centres (per-lane partial sums); the
+    // primitive sums of record o live in lane o (o < 64) or lane o - 64
+    double gx0[3] = {0.0, 0.0, 0.0}, gx1[3] = {0.0, 0.0, 0.0};
+    double pe0[2] = {0.0, 0.0}, pe1[2] = {0.0, 0.0}, pv[2] = {0.0, 0.0};
     for (int it = 0; it < niter; it++) {
         const int kk = it * S::NT + team;
         const bool valid = kk < t.nket;
-        const int ketp = t.ket0 + (valid ? kk : 0);
-        const GPair ket = gload_pair(B, ketp);
+        const int otp = t.ket0 + (valid ? kk : 0);
+        const GPair ot = gload_pair(B, otp);
+        const GPair &br = SIDE == 0 ? sd : ot;  // canonical bra (a b|
+        const GPair &kt = SIDE == 0 ? ot : sd;  // canonical ket |c d)
         double s = valid ? 1.0 : 0.0;
-        if (bra.sa == bra.sb) s *= 0.5;
-        if (ket.sa == ket.sb) s *= 0.5;
-        if (t.bra == ketp) s *= 0.5;
+        if (br.sa == br.sb) s *= 0.5;
+        if (kt.sa == kt.sb) s *= 0.5;
+        if (t.bra == otp) s *= 0.5;
         // this lane's outputs: energy weight, packed g indices, powers of the
         // two differentiated centres (2 bits per axis)
         double W[S::NPL];
@@ -183,7 +196,7 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
                 const int fa = f % C::NFA, t1 = f / C::NFA;
                 const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
                 const int fc = t2 % C::NFC, fd = t2 / C::NFC;
-                const size_t a = bra.aoa + fa, b = bra.aob + fb, c = ket.aoa + fc, d = ket.aob + fd;
+                const size_t a = br.aoa + fa, b = br.aob + fb, c = kt.aoa + fc, d = kt.aob + fd;
                 W[k] = s * (4.0 * D[a * n + b] * D[c * n + d] - D[a * n + c] * D[b * n + d] - D[a * n + d] * D[b * n + c]);
                 int packed = 0, pws = 0;
 #pragma unroll
@@ -198,11 +211,7 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
                 pw[k] = pws;
             }
         }
-        const GPair &sd = SIDE == 0 ? bra : ket;  // pair whose centres are differentiated
-        const GPair &ot = SIDE == 0 ? ket : bra;
         const double rxy[3] = {sd.rb[0] - ot.rb[0], sd.rb[1] - ot.rb[1], sd.rb[2] - ot.rb[2]};
-        const int2 si0 = shell_info[sd.sa], si1 = shell_info[sd.sb];
-        double gx0[3] = {0.0, 0.0, 0.0}, gx1[3] = {0.0, 0.0, 0.0};
         for (int o = 0; o < sd.npp; o++) {
             const PrimPair po = B.pp[sd.off + o];
             const double al0 = po.pad0, al1 = po.pad1;  // exponents of the two centres
@@ -239,10 +248,22 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
                 }
                 __syncthreads();
             }
-            e0 = team_sum<S::TS>(e0);
-            e1 = team_sum<S::TS>(e1);
-            val = team_sum<S::TS>(val);
-            if (tl == 0 && valid) {
+            e0 = team_sum<64>(e0);
+            e1 = team_sum<64>(e1);
+            val = team_sum<64>(val);
+            if (o < 64) {
+                if (lane == o) {
+                    pe0[0] += e0;
+                    pe1[0] += e1;
+                    pv[0] += val;
+                }
+            } else if (o < 128) {
+                if (lane == o - 64) {
+                    pe0[1] += e0;
+                    pe1[1] += e1;
+                    pv[1] += val;
+                }
+            } else if (lane == 0) {
                 const int2 ij = pp_idx[sd.off + o];
                 const int k0 = si0.y + ij.x, k1 = si1.y + ij.y;
                 atomicAdd(&dprim[2 * k0], e0);
@@ -251,14 +272,26 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
                 atomicAdd(&dprim[2 * k1 + 1], val / coef[k1]);
             }
         }
+    }
 #pragma unroll
-        for (int x = 0; x < 3; x++) {
-            const double v0 = team_sum<S::TS>(gx0[x]);
-            const double v1 = team_sum<S::TS>(gx1[x]);
-            if (tl == 0 && valid) {
-                atomicAdd(&gatom[3 * si0.x + x], v0);
-                atomicAdd(&gatom[3 * si1.x + x], v1);
-            }
+    for (int x = 0; x < 3; x++) {
+        const double v0 = team_sum<64>(gx0[x]);
+        const double v1 = team_sum<64>(gx1[x]);
+        if (lane == 0) {
+            atomicAdd(&gatom[3 * si0.x + x], v0);
+            atomicAdd(&gatom[3 * si1.x + x], v1);
+        }
+    }
+#pragma unroll
+    for (int slot = 0; slot < 2; slot++) {
+        const int o = 64 * slot + lane;
+        if (o < sd.npp) {
+            const int2 ij = pp_idx[sd.off + o];
+            const int k0 = si0.y + ij.x, k1 = si1.y + ij.y;
+            atomicAdd(&dprim[2 * k0], pe0[slot]);
+            atomicAdd(&dprim[2 * k1], pe1[slot]);
+            atomicAdd(&dprim[2 * k0 + 1], pv[slot] / coef[k0]);
+            atomicAdd(&dprim[2 * k1 + 1], pv[slot] / coef[k1]);
         }
     }
 }
