@@ -154,13 +154,15 @@ struct PrimPair {
 
 // Reciprocal and reciprocal square root for positive normal arguments:
 // hardware estimate plus two Newton steps (relative error ~1 ulp).
+// 1/sqrt(x) for x > 0: the hardware estimate (relative error about 5e-8)
+// refined by one third-order step, y (1 + e/2 + 3e^2/8) with e = 1 - x y^2;
+// the dropped term 5e^3/16 is below 1e-21, so the result is within about
+// one ulp, like two Newton steps but with three fewer operations.
 __device__ __forceinline__ double rsqrt_pos(double x)
 {
-    double y = __builtin_amdgcn_rsq(x);
-    double r = fma(-x * y, y, 1.0);
-    y = fma(0.5 * y, r, y);
-    r = fma(-x * y, y, 1.0);
-    return fma(0.5 * y, r, y);
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y, e * fma(0.375, e, 0.5), y);
 }
 __device__ __forceinline__ double rcp_pos(double x)
 {
@@ -265,10 +267,11 @@ struct RysLds {
 };
 
 // One primitive quartet, all roots, in working precision T with the Rys
-// table block in LDS; accumulated into eri.  rbd = Rb - Rd, rab = Rb - Ra,
+// table block in LDS; accumulated into eri.  qd = (Q - Rd) - (Rb - Rd) =
+// Q - Rb of the ket primitive pair (P - Q = (P - Rb) - qd), rab = Rb - Ra,
 // rcd = Rd - Rc.
 template <int LA, int LB, int LC, int LD, typename T>
-__device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, const PrimPair &q, const T rbd[3],
+__device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, const PrimPair &q, const T qd[3],
                                                  const T rab[3], const T rcd[3], const T *rt)
 {
     using C = Class<LA, LB, LC, LD>;
@@ -276,7 +279,7 @@ __device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, cons
     const T rsq = rsqrt_t(aij + akl);
     const T rs = rsq * rsq;
     const T a0 = aij * akl * rs;
-    const T pq[3] = {(T)p.pbx - (T)q.pbx + rbd[0], (T)p.pby - (T)q.pby + rbd[1], (T)p.pbz - (T)q.pbz + rbd[2]};
+    const T pq[3] = {(T)p.pbx - qd[0], (T)p.pby - qd[1], (T)p.pbz - qd[2]};
     const T rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
     const T fac1 = (T)p.K * (T)q.K * rsq;
     T v[C::NROOTS], w[C::NROOTS];

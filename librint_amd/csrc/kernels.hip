@@ -115,20 +115,22 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
         for (int ik = 0; ik < knpp; ik++) {
             const PrimPair q = qn;
             if (ik + 1 < knpp) qn = B.pp[ket.off + ik + 1];
+            const T qd[3] = {(T)q.pbx - rbd[0], (T)q.pby - rbd[1], (T)q.pbz - rbd[2]};
             PrimPair pn = B.pp[boff];
 #pragma unroll 2
             for (int ib = 0; ib < bnpp; ib++) {
                 const PrimPair p = pn;
                 if (ib + 1 < bnpp) pn = B.pp[boff + ib + 1];
-                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, rbd, rab, rcd, rt);
+                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, qd, rab, rcd, rt);
             }
         }
     } else {
         for (int ik = 0; ik < knpp; ik++) {
             const PrimPair q = B.pp[ket.off + ik];
+            const T qd[3] = {(T)q.pbx - rbd[0], (T)q.pby - rbd[1], (T)q.pbz - rbd[2]};
             for (int ib = 0; ib < bnpp; ib++) {
                 const PrimPair p = B.pp[boff + ib];
-                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, rbd, rab, rcd, rt);
+                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, qd, rab, rcd, rt);
             }
         }
     }
