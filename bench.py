@@ -229,8 +229,11 @@ def main():
     ms_per_step = wall / args.steps * 1e3
     quartets = st["quartets"]
     jk_ms = float(np.mean(ev_ms))
-    # dominant kernel: the class launch with the largest device time
-    dom = max(cstats, key=lambda c: c["ms"])
+    # dominant kernel: the class launch with the most algorithmic flops.  The
+    # library runs it first and alone, so its event time is its own; the
+    # other launches then overlap on several streams and their times include
+    # the sharing.
+    dom = max(cstats, key=lambda c: c["flops"])
     dom_achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
     achieved_all = st["flops"] / (jk_ms * 1e-3) / 1e12
     line = {
@@ -250,7 +253,8 @@ def main():
         "config": {"workload": args.workload, "nbas": m.nbas, "nao": m.nao(), "schwarz": tau,
                    "precision": "fp64", "quartets": int(quartets), "prim_quartets": int(st["prim_quartets"]),
                    "integrals": int(st["integrals"]), "candidates": int(st["candidates"]),
-                   "launches": int(st["launches"])},
+                   "launches": int(st["launches"]),
+                   "streams": int(os.environ.get("LIBRINT_STREAMS", "2"))},
         "roofline": {"bound": "mfma", "achieved": dom_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": dom_achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC.get((args.workload, args.precision, dom["kernel"])),

@@ -94,6 +94,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     if (const char *e = getenv("LIBRINT_DROW")) drow_mode_ = atoi(e) != 0 ? 1 : 0;
     check(hipEventCreate(&ev0_), "hipEventCreate");
     check(hipEventCreate(&ev1_), "hipEventCreate");
+    if (const char *e = getenv("LIBRINT_STREAMS")) nstreams_ = std::max(1, std::min(8, atoi(e)));
     build_pairs();
     upload();
     compute_schwarz();
@@ -129,6 +130,10 @@ Context::~Context()
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evj_) (void)hipEventDestroy(e);
+    if (evf_) (void)hipEventDestroy(evf_);
+    for (hipStream_t q : side_) (void)hipStreamDestroy(q);
 }
 
 DevBasis Context::devb() const
@@ -449,6 +454,9 @@ void Context::build_tasks()
         if (L.count) launches_.push_back(L);
     }
     stats_.launches = (int)launches_.size();
+    dominant_ = 0;
+    for (size_t k = 1; k < launches_.size(); k++)
+        if (launches_[k].flops > launches_[dominant_].flops) dominant_ = (int)k;
 
     // Coulomb slabs: every launch of a register class stores its ket-side
     // Coulomb blocks per task; items = (launch, 64-ket window) with the list
@@ -460,6 +468,7 @@ void Context::build_tasks()
     // under JBUF_WHOLE, else the buffer holds one launch at a time
     const size_t JBUF_LAUNCH = (size_t)1 << 30;  // 8 GiB
     const size_t JBUF_WHOLE = (size_t)1 << 28;   // 2 GiB
+    const size_t JBUF_STREAMS = (size_t)1 << 31; // 16 GiB: per-stream buffers of the per-launch mode
     size_t total = 0, largest = 0;
     max_nfcd_ = 1;
     for (ClassLaunch &L : launches_) {
@@ -505,7 +514,13 @@ void Context::build_tasks()
     dev_free(d_item_tasks_);
     d_items_ = dev_upload(items);
     d_item_tasks_ = dev_upload(item_tasks);
-    const size_t need = jslab_whole_ ? total : largest;
+    // per-launch mode: one buffer per stream, so that launches on different
+    // streams never share slab memory
+    slab_streams_ = 1;
+    if (!jslab_whole_ && largest > 0)
+        slab_streams_ = (int)std::max<size_t>(1, std::min<size_t>((size_t)nstreams_, JBUF_STREAMS / largest));
+    jslot_ = largest;
+    const size_t need = jslab_whole_ ? total : largest * slab_streams_;
     if (need > nJ_) {
         dev_free(d_J_);
         check(hipMalloc(&d_J_, sizeof(double) * need), "hipMalloc");
@@ -515,8 +530,11 @@ void Context::build_tasks()
         fprintf(stderr, "librint: coulomb slabs %zu doubles (%s), %d reduction items\n", need,
                 jslab_whole_ ? "one reduction" : "per launch", nitems_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
     evc_.assign(launches_.size(), nullptr);
+    evs_.assign(launches_.size(), nullptr);
     for (auto &e : evc_) check(hipEventCreate(&e), "hipEventCreate");
+    for (auto &e : evs_) check(hipEventCreate(&e), "hipEventCreate");
     dev_free(d_tasks_);
     d_tasks_ = dev_upload(tasks);
     dev_free(d_groups_);
@@ -603,8 +621,37 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     const unsigned char *flags = classify ? d_flags_ : nullptr;
     const unsigned char *gmask = classify ? d_gmask_ : nullptr;
     check(hipEventRecord(ev0_, st), "hipEventRecord");
-    for (size_t k = 0; k < launches_.size(); k++) {
+    // The launch with the most flops runs first and alone on st: it fills the
+    // device by itself and its timing stays its own.  The others are then
+    // dealt round-robin over st and ns - 1 extra streams; they share nothing
+    // but M (atomics) and, in the per-launch slab mode, their stream's slab
+    // buffer.
+    const int nl = (int)launches_.size();
+    int ns = std::min(nstreams_, nl - 1);
+    if (!jslab_whole_) ns = std::min(ns, slab_streams_);
+    ns = std::max(ns, 1);
+    if (ns > 1 && (int)side_.size() < ns - 1) {
+        while ((int)side_.size() < ns - 1) {
+            hipStream_t q = nullptr;
+            check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "hipStreamCreate");
+            side_.push_back(q);
+            hipEvent_t e = nullptr;
+            check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+            evj_.push_back(e);
+        }
+        if (!evf_) check(hipEventCreateWithFlags(&evf_, hipEventDisableTiming), "hipEventCreate");
+    }
+    for (int j = 0; j < nl; j++) {
+        const int k = j == 0 ? dominant_ : (j <= dominant_ ? j - 1 : j);
         ClassLaunch &L = launches_[k];
+        if (j == 1 && ns > 1) {
+            check(hipEventRecord(evf_, st), "hipEventRecord");
+            for (int i = 0; i < ns - 1; i++) check(hipStreamWaitEvent(side_[i], evf_, 0), "hipStreamWaitEvent");
+        }
+        const int slot = j == 0 ? 0 : (j - 1) % ns;
+        const hipStream_t sk = slot == 0 ? st : side_[slot - 1];
+        double *jbuf = jslab_whole_ ? d_J_ : d_J_ + (size_t)slot * jslot_;
+        check(hipEventRecord(evs_[k], sk), "hipEventRecord");
         int *ctr = d_counters_ + 2 * k;
         const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, false);
         const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
@@ -615,21 +662,25 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         drow = drow && grouped && jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, true) <= (size_t)max_lds_;
         L.drow = drow ? 1 : 0;
         L.variant = grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2);
-        double *js = L.jbase >= 0 ? d_J_ + L.jbase : nullptr;
+        double *js = L.jbase >= 0 ? jbuf + L.jbase : nullptr;
         if (grouped)
             check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
                                   gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, js, L.first, drow,
-                                  st),
+                                  sk),
                   "jk group launch");
         else
             check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + L.first, L.count, ctr,
-                            flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_, nao_, js, 0, st),
+                            flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_, nao_, js, 0, sk),
                   "jk launch");
         if (!jslab_whole_ && L.icount)
-            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, 1, d_J_, B,
-                                 d_M_, nao_, st),
+            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, 1, jbuf, B,
+                                 d_M_, nao_, sk),
                   "coulomb reduction launch");
-        check(hipEventRecord(evc_[k], st), "hipEventRecord");
+        check(hipEventRecord(evc_[k], sk), "hipEventRecord");
+    }
+    for (int i = 0; i < ns - 1; i++) {
+        check(hipEventRecord(evj_[i], side_[i]), "hipEventRecord");
+        check(hipStreamWaitEvent(st, evj_[i], 0), "hipStreamWaitEvent");
     }
     if (jslab_whole_ && nitems_)
         check(launch_jreduce(d_items_, nitems_, max_nfcd_, d_item_tasks_, d_tasks_, flags, 1, d_J_, B, d_M_, nao_, st),
@@ -652,11 +703,9 @@ void Context::fock_host(const double *D, double *G)
 
 void Context::collect_class_times()
 {
-    hipEvent_t prev = ev0_;
     for (size_t k = 0; k < launches_.size(); k++) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, prev, evc_[k]) == hipSuccess) launches_[k].ms = ms;
-        prev = evc_[k];
+        if (hipEventElapsedTime(&ms, evs_[k], evc_[k]) == hipSuccess) launches_[k].ms = ms;
     }
 }
 

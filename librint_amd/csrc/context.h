@@ -173,6 +173,18 @@ private:
     void build_grad_tasks();
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<hipEvent_t> evc_;  // one event after each class launch
+    std::vector<hipEvent_t> evs_;  // one event before each class launch (same stream)
+    // class launches after the dominant one are dealt round-robin over the
+    // caller's stream and nstreams_ - 1 extra streams (LIBRINT_STREAMS),
+    // joined before the Coulomb reduction: small launches and launch tails
+    // overlap
+    int nstreams_ = 2;
+    int dominant_ = 0;      // launch with the most flops: runs first, alone
+    int slab_streams_ = 1;  // per-launch slab mode: buffers of jslot_ doubles, one per stream
+    size_t jslot_ = 0;
+    std::vector<hipStream_t> side_;
+    std::vector<hipEvent_t> evj_;  // join events, one per extra stream
+    hipEvent_t evf_ = nullptr;     // fork event
     DevBasis devb() const;
 };
 
