@@ -368,13 +368,8 @@ __host__ __device__ constexpr int RysLdsDoubles()
 // gmask: per group, bit f set when the group holds a task with flag f.
 // DROW: the bra's density rows (NFA + NFB rows of nao) are staged in LDS
 // beside the exchange rows, so the exchange terms read D from LDS.
-#ifdef LIBRINT_WAVES_PER_EU
-#define GROUP_OCC __attribute__((amdgpu_waves_per_eu(LIBRINT_WAVES_PER_EU)))
-#else
-#define GROUP_OCC
-#endif
 template <int LA, int LB, int LC, int LD, typename T, bool DROW>
-__global__ __launch_bounds__(256) GROUP_OCC void jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
+__global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
                                                        const JKTask *__restrict__ tasks, int *__restrict__ counter,
                                                        const unsigned char *__restrict__ flags,
                                                        const unsigned char *__restrict__ gmask, int accept, DevBasis B,
