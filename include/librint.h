@@ -34,12 +34,12 @@ void free_c(double *ptr, size_t len);
 
 /* n*n one-electron matrix, R[mu*n+nu]; coord 0 cartesian (n = nao), 1 real
  * spherical (n = sum of 2l+1, transform of src/cart2sph.rs:88380-88530);
- * typec 0 overlap, 1 kinetic, 2 nuclear.  Computed on the GPU, l <= 2.
+ * typec 0 overlap, 1 kinetic, 2 nuclear.  Computed on the GPU, l <= 4.
  * reference src/p2c.rs:85-100, src/scf.rs:160-213 */
 double *int1e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, int coord, int typec);
 
-/* n^4 ERI tensor R[((i*n+j)*n+k)*n+l] = (ij|kl), computed on the GPU; coord 0
- * cartesian (n = nao), 1 real spherical.  reference src/p2c.rs:108-123,
+/* n^4 ERI tensor R[((i*n+j)*n+k)*n+l] = (ij|kl), computed on the GPU (l <= 4);
+ * coord 0 cartesian (n = nao), 1 real spherical.  reference src/p2c.rs:108-123,
  * src/scf.rs:215-311 */
 double *int2e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, int coord);
 
@@ -60,7 +60,8 @@ double scf_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t
  * src/utils.rs:152-175: every primitive exponent and contraction
  * coefficient), P the density they are evaluated at; release with free_c.
  * The two-electron part runs on the GPU (adjoint Rys kernel), the
- * one-electron parts on the host.
+ * one-electron parts on the host; shells with l <= 2 (NULL and an error
+ * message for l > 2).
  *   dS_u          nao*nao*(s2-s1), [(a*nao+b)*L + k] = dS_ab/denv[s1+k]
  *                 reference src/p2c.rs:130-144, src/dscf.rs:96-150
  *   grad_c        dHcore + dR            reference src/p2c.rs:240-258, src/dscf.rs:547-557
@@ -137,7 +138,8 @@ int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
  * la, lb, lc, ld, quartets, primitive quartets, algorithmic flops, lanes
  * launched (64 x tasks), device ms of the launch in the last host-side build
  * (librint_fock), task count, FP32 recurrences (0/1), kernel of the last
- * build (0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, -1 none yet),
+ * build (0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel for
+ * classes with an f or g shell, -1 none yet),
  * bra density rows staged in LDS (0/1).
  * Returns the number of launches (out may be NULL to query it); writes at
  * most n launches. */
@@ -154,6 +156,7 @@ int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);
  *                  contraction coefficient (dHcore + dR - dS/2)
  * parts (optional): [gH | gR | gS | gN] (3*natm each) then [dH | dR | dS]
  * (env_l each).  Quartets are screened with the context's Schwarz threshold.
+ * Shells with l <= 2 (returns -1 otherwise).
  * No counterpart in the reference (its gradients are basis-parameter only). */
 int librint_ctx_gradient(librint_ctx *ctx, const double *P, double *gnuc, double *denv, double *parts);
 

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIBDIR = os.path.join(HERE, "lib")
 SO = os.path.join(LIBDIR, "librint.so")
-SOURCES = ["kernels.hip", "grad.hip", "mlp.hip", "int1e_kernel.hip", "int1e_gpu.cpp", "cart2sph.hip", "context.cpp", "int1e.cpp", "scf.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "hl.hip", "grad.hip", "mlp.hip", "int1e_kernel.hip", "int1e_gpu.cpp", "cart2sph.hip", "context.cpp", "int1e.cpp", "scf.cpp", "capi.cpp"]
 ARCH = os.environ.get("LIBRINT_OFFLOAD_ARCH", "gfx950")
 
 

@@ -108,6 +108,8 @@ class FockContext:
                 r["kernel"] = "jk_group_kernel<%s, %s, %s>" % (ls, t, "true" if r["drow"] else "false")
             elif v == 0:
                 r["kernel"] = "jk_kernel<%s, %s>" % (ls, t)
+            elif v == 3:
+                r["kernel"] = "hl_kernel<3>"  # run-time class shape (an f or g shell)
             else:
                 r["kernel"] = None
         return rows

@@ -1,5 +1,5 @@
 // Cartesian -> real spherical transform of integral matrices and tensors
-// (l <= 2), the transform of reference src/cart2sph.rs:88380-88530.
+// (l <= 4), the transform of reference src/cart2sph.rs:88380-88530.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,16 +9,17 @@
 
 namespace librint {
 
-// One spherical function: up to 3 cartesian components with coefficients.
+// One spherical function: up to 6 cartesian components with coefficients.
+constexpr int C2S_MAXTERM = 6;
 struct C2SRow {
     int n;
-    int cart[3];
-    double c[3];
+    int cart[C2S_MAXTERM];
+    double c[C2S_MAXTERM];
 };
 
 // Rows for every spherical function of a basis (bas in libcint layout: l and
 // nctr per shell), in spherical AO order; ncart receives the cartesian count.
-// Throws for l > 2.
+// Throws for l > 4.
 std::vector<C2SRow> c2s_rows(const int *bas, size_t bas_l, int *ncart);
 
 // out[sph^ndim] = sum over cartesian components of prod_k c_k * in[cart^ndim]

@@ -5,6 +5,7 @@
 #include "context.h"
 
 #include "grad.h"
+#include "hl.h"
 
 #include <algorithm>
 #include <cmath>
@@ -22,19 +23,23 @@ void check(hipError_t e, const char *what)
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// pair classes (la <= lb), lb major: (00) (01) (11) (02) (12) (22) (03) ...
+// (33) (04) ... (44); classes 0-5 have their own kernels, 6-14 (an f or g
+// shell) use the run-time shaped kernels of hl.hip
 int cls_of(int la, int lb)
 {
-    static const int tab[3][3] = {{0, 1, 3}, {-1, 2, 4}, {-1, -1, 5}};
-    if (la > lb || lb > 2) return -1;
-    return tab[la][lb];
+    if (la < 0 || la > lb || lb > HL_LMAX) return -1;
+    return lb * (lb + 1) / 2 + la;
 }
 
 static void cls_ls(int c, int &la, int &lb)
 {
-    static const int A[6] = {0, 0, 1, 0, 1, 2}, Bv[6] = {0, 1, 1, 2, 2, 2};
-    la = A[c];
-    lb = Bv[c];
+    lb = 0;
+    while ((lb + 1) * (lb + 2) / 2 <= c) lb++;
+    la = c - lb * (lb + 1) / 2;
 }
+
+static constexpr int NCLS = (HL_LMAX + 1) * (HL_LMAX + 2) / 2;
 
 static double common_fac_sp(int l)
 { // reference src/g1e.rs:776-782
@@ -70,7 +75,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     for (int s = 0; s < nb; s++) {
         const int *b = &bas_[8 * s];
         if (b[0] < 0 || (size_t)b[0] >= atm_l / 6) throw std::invalid_argument("bas atom index out of range");
-        if (b[1] < 0 || b[1] > 2) throw std::invalid_argument("angular momentum l > 2 is not supported by the GPU path");
+        if (b[1] < 0 || b[1] > HL_LMAX) throw std::invalid_argument("angular momentum l > 4 is not supported by the GPU path");
         // general contraction: one segmented virtual shell per contracted function
         for (int c = 0; c < b[3]; c++) {
             Shell sh;
@@ -127,6 +132,7 @@ Context::~Context()
     dev_free(d_shell_slot_);
     dev_free(d_slot_coef_);
     dev_free(d_gtasks1_);
+    dev_free(d_hlscr_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
@@ -239,8 +245,8 @@ void Context::compute_schwarz()
     // diagonal (ij|ij) per pair on the GPU, grouped by class
     const int np = (int)pairs_.size();
     std::vector<ListTask> tasks;
-    std::vector<std::pair<int, int>> ranges(6, {0, 0});
-    for (int c = 0; c < 6; c++) {
+    std::vector<std::pair<int, int>> ranges(NCLS, {0, 0});
+    for (int c = 0; c < NCLS; c++) {
         ranges[c].first = (int)tasks.size();
         for (int p = 0; p < np; p++)
             if (pairs_[p].cls == c) tasks.push_back(ListTask{p, p, (long long)p});
@@ -250,11 +256,11 @@ void Context::compute_schwarz()
     double *dq = nullptr;
     check(hipMalloc(&dq, sizeof(double) * std::max(np, 1)), "hipMalloc");
     const DevBasis B = devb();
-    for (int c = 0; c < 6; c++) {
+    for (int c = 0; c < NCLS; c++) {
         if (!ranges[c].second) continue;
         int la, lb;
         cls_ls(c, la, lb);
-        check(launch_list(la, lb, la, lb, dt + ranges[c].first, ranges[c].second, B, dq, 1, nao_, 0), "schwarz launch");
+        check(list_launch(la, lb, la, lb, dt + ranges[c].first, ranges[c].second, B, dq, 1, 0), "schwarz launch");
     }
     std::vector<double> Q(np);
     check(hipMemcpy(Q.data(), dq, sizeof(double) * np, hipMemcpyDeviceToHost), "hipMemcpy D2H");
@@ -284,6 +290,27 @@ void Context::compute_schwarz()
         bins_.back().count++;
     }
     upload();
+}
+
+double *Context::hl_scratch(const HLShape &S)
+{
+    // one contracted block per workgroup; up to 8192 workgroups, at most
+    // 2^26 doubles (512 MiB)
+    const size_t want = std::min<size_t>((size_t)1 << 26, (size_t)S.nf * 8192);
+    if (want > nhlscr_) {
+        dev_free(d_hlscr_);
+        check(hipMalloc(&d_hlscr_, sizeof(double) * want), "hipMalloc");
+        nhlscr_ = want;
+    }
+    return d_hlscr_;
+}
+
+hipError_t Context::list_launch(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
+                                double *out, int mode, hipStream_t st)
+{
+    if (!hl_class(la, lb, lc, ld)) return launch_list(la, lb, lc, ld, tasks, ntask, B, out, mode, nao_, st);
+    const HLShape S = hl_shape(la, lb, lc, ld);
+    return launch_hl_list(S, tasks, ntask, B, out, mode, nao_, hl_scratch(S), nhlscr_, st);
 }
 
 void Context::set_schwarz(double tau)
@@ -349,7 +376,7 @@ void Context::build_tasks()
                 const Bin &by = bins_[Y];
                 int lc, ld;
                 cls_ls(by.cls, lc, ld);
-                if (!class_supported(la, lb, lc, ld))
+                if (!class_supported(la, lb, lc, ld) && !hl_class(la, lb, lc, ld))
                     throw std::runtime_error("angular class not supported by the GPU kernels");
                 // kets of bin Y with Q_p Q_k > tau: a prefix (Q descending)
                 int lo = 0, hi = by.count;
@@ -648,7 +675,9 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
             check(hipEventRecord(evf_, st), "hipEventRecord");
             for (int i = 0; i < ns - 1; i++) check(hipStreamWaitEvent(side_[i], evf_, 0), "hipStreamWaitEvent");
         }
-        const int slot = j == 0 ? 0 : (j - 1) % ns;
+        // launches of the l > 2 classes share one scratch buffer: they all
+        // stay on st
+        const int slot = (j == 0 || hl_class(L.la, L.lb, L.lc, L.ld)) ? 0 : (j - 1) % ns;
         const hipStream_t sk = slot == 0 ? st : side_[slot - 1];
         double *jbuf = jslab_whole_ ? d_J_ : d_J_ + (size_t)slot * jslot_;
         check(hipEventRecord(evs_[k], sk), "hipEventRecord");
@@ -661,9 +690,15 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         bool drow = drow_mode_ == 1;
         drow = drow && grouped && jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, true) <= (size_t)max_lds_;
         L.drow = drow ? 1 : 0;
-        L.variant = grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2);
+        const bool hl = hl_class(L.la, L.lb, L.lc, L.ld);
+        L.variant = hl ? 3 : (grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2));
         double *js = L.jbase >= 0 ? jbuf + L.jbase : nullptr;
-        if (grouped)
+        if (hl) {
+            const HLShape S = hl_shape(L.la, L.lb, L.lc, L.ld);
+            check(launch_hl_jk(S, d_tasks_ + L.first, L.count, flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_,
+                               nao_, hl_scratch(S), nhlscr_, sk),
+                  "jk launch (l > 2)");
+        } else if (grouped)
             check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
                                   gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, js, L.first, drow,
                                   sk),
@@ -747,7 +782,7 @@ void Context::tensor_device(double *dR)
         cls_ls(kv.first.second, lc, ld);
         ListTask *dt = dev_upload(kv.second);
         bufs.push_back(dt);
-        check(launch_list(la, lb, lc, ld, dt, (int)kv.second.size(), B, dR, 2, nao_, 0), "tensor launch");
+        check(list_launch(la, lb, lc, ld, dt, (int)kv.second.size(), B, dR, 2, 0), "tensor launch");
     }
     check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     for (auto p : bufs) dev_free(p);
@@ -820,7 +855,7 @@ size_t Context::shells_host(const int *quartets, int nq, double *out)
             cls_ls(kv.first.second, lc, ld);
             ListTask *dt = dev_upload(kv.second);
             bufs.push_back(dt);
-            check(launch_list(la, lb, lc, ld, dt, (int)kv.second.size(), B, dout, 0, nao_, 0), "shells launch");
+            check(list_launch(la, lb, lc, ld, dt, (int)kv.second.size(), B, dout, 0, 0), "shells launch");
         }
         check(hipMemcpy(blocks.data(), dout, sizeof(double) * dev_total, hipMemcpyDeviceToHost), "hipMemcpy D2H");
         for (auto p : bufs) dev_free(p);
@@ -916,6 +951,8 @@ void Context::build_grad_tasks()
 
 void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
 {
+    for (const Shell &S : shells_)
+        if (S.l > 2) throw std::invalid_argument("analytic gradients cover l <= 2");
     // per-shell primitive slots: exponent and coefficient derivatives are
     // accumulated per (shell, primitive) and folded onto env entries here
     const int ns = (int)shells_.size();

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "eri_core.h"
+#include "hl.h"
 #include "kernels.h"
 
 namespace librint {
@@ -40,7 +41,7 @@ struct ClassLaunch {
     long long quartets = 0, prim_quartets = 0, lanes = 0;
     double flops = 0;
     float ms = 0.f;    // device time of this launch in the last build
-    int variant = -1;  // kernel of the last build: 0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel
+    int variant = -1;  // kernel of the last build: 0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel
     long long jbase = -1;  // Coulomb slab offset (-1: ket Coulomb block by global atomics)
     int ifirst = 0, icount = 0;  // slab reduction items of this launch
     int drow = 0;                // bra density rows staged in LDS in the last build
@@ -171,6 +172,13 @@ private:
     JKTask *d_gtasks1_ = nullptr;
     bool gtasks_valid_ = false;
     void build_grad_tasks();
+    // classes with an f or g shell (hl.hip): scratch of one contracted block
+    // per workgroup, allocated on first use
+    double *d_hlscr_ = nullptr;
+    size_t nhlscr_ = 0;
+    double *hl_scratch(const HLShape &S);
+    hipError_t list_launch(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
+                           double *out, int mode, hipStream_t st);
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<hipEvent_t> evc_;  // one event after each class launch
     std::vector<hipEvent_t> evs_;  // one event before each class launch (same stream)

@@ -270,11 +270,44 @@ struct RysLds {
 // table block in LDS; accumulated into eri.  qd = (Q - Rd) - (Rb - Rd) =
 // Q - Rb of the ket primitive pair (P - Q = (P - Rb) - qd), rab = Rb - Ra,
 // rcd = Rd - Rc.
+// Contribution of one Rys root (node v = t^2, weight w) of one primitive
+// quartet: recurrence coefficients, VRR, HRR and the gout products.
+template <int LA, int LB, int LC, int LD, typename T>
+__device__ __forceinline__ void root_term(T *eri, const PrimPair &p, const PrimPair &q, const T pq[3], T aij, T akl,
+                                          T rs, T fac1, T vr, T wr, const T rab[3], const T rcd[3])
+{
+    using C = Class<LA, LB, LC, LD>;
+    const T tr = vr * rs;
+    const T b00 = (T)0.5 * tr;
+    const T b10 = (T)p.h * ((T)1 - akl * tr);
+    const T b01 = (T)q.h * ((T)1 - aij * tr);
+    const T ta = akl * tr;
+    const T tc = aij * tr;
+    T gx[C::GS], gy[C::GS], gz[C::GS];
+    gx[0] = (T)1;
+    gy[0] = (T)1;
+    gz[0] = wr * fac1;
+    vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gx, (T)p.pbx - ta * pq[0], (T)q.pbx + tc * pq[0], b10, b01, b00);
+    vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gy, (T)p.pby - ta * pq[1], (T)q.pby + tc * pq[1], b10, b01, b00);
+    vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gz, (T)p.pbz - ta * pq[2], (T)q.pbz + tc * pq[2], b10, b01, b00);
+    if constexpr (LA > 0 || LC > 0) {
+        hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gx, rab[0], rcd[0]);
+        hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gy, rab[1], rcd[1]);
+        hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gz, rab[2], rcd[2]);
+    }
+    gout_acc<LA, LB, LC, LD>(eri, gx, gy, gz);
+}
+
+// One primitive quartet, all roots, in working precision T with the Rys
+// table block in LDS; accumulated into eri.  qd = (Q - Rd) - (Rb - Rd) =
+// Q - Rb of the ket primitive pair (P - Q = (P - Rb) - qd), rab = Rb - Ra,
+// rcd = Rd - Rc.
 template <int LA, int LB, int LC, int LD, typename T>
 __device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, const PrimPair &q, const T qd[3],
                                                  const T rab[3], const T rcd[3], const T *rt)
 {
     using C = Class<LA, LB, LC, LD>;
+    constexpr int N = C::NROOTS;
     const T aij = (T)p.e, akl = (T)q.e;
     const T rsq = rsqrt_t(aij + akl);
     const T rs = rsq * rsq;
@@ -282,30 +315,10 @@ __device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, cons
     const T pq[3] = {(T)p.pbx - qd[0], (T)p.pby - qd[1], (T)p.pbz - qd[2]};
     const T rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
     const T fac1 = (T)p.K * (T)q.K * rsq;
-    T v[C::NROOTS], w[C::NROOTS];
-    rys_nodes_lds<T, C::NROOTS, RysLds<LA, LB, LC, LD, T>::WONLY>(a0 * rr, rt, v, w);
+    T v[N], w[N];
+    rys_nodes_lds<T, N, RysLds<LA, LB, LC, LD, T>::WONLY>(a0 * rr, rt, v, w);
 #pragma unroll
-    for (int r = 0; r < C::NROOTS; r++) {
-        const T tr = v[r] * rs;
-        const T b00 = (T)0.5 * tr;
-        const T b10 = (T)p.h * ((T)1 - akl * tr);
-        const T b01 = (T)q.h * ((T)1 - aij * tr);
-        const T ta = akl * tr;
-        const T tc = aij * tr;
-        T gx[C::GS], gy[C::GS], gz[C::GS];
-        gx[0] = (T)1;
-        gy[0] = (T)1;
-        gz[0] = w[r] * fac1;
-        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gx, (T)p.pbx - ta * pq[0], (T)q.pbx + tc * pq[0], b10, b01, b00);
-        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gy, (T)p.pby - ta * pq[1], (T)q.pby + tc * pq[1], b10, b01, b00);
-        vrr_axis<C::NMAX, C::MMAX, C::DL, C::DJ>(gz, (T)p.pbz - ta * pq[2], (T)q.pbz + tc * pq[2], b10, b01, b00);
-        if constexpr (LA > 0 || LC > 0) {
-            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gx, rab[0], rcd[0]);
-            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gy, rab[1], rcd[1]);
-            hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gz, rab[2], rcd[2]);
-        }
-        gout_acc<LA, LB, LC, LD>(eri, gx, gy, gz);
-    }
+    for (int r = 0; r < N; r++) root_term<LA, LB, LC, LD, T>(eri, p, q, pq, aij, akl, rs, fac1, v[r], w[r], rab, rcd);
 }
 
 // One primitive quartet, all roots, accumulated into eri (double precision).

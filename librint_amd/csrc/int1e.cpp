@@ -150,6 +150,9 @@ template <typename Visit>
 static void visit_bra_derivs(const Int1eInput &in, int typ, Visit f)
 {
     const int ns = (int)in.shells.size();
+    // the 1D tables below hold the bra momentum raised by 2 for l <= 2
+    for (const Shell1 &S : in.shells)
+        if (S.l < 0 || S.l > 2) throw std::invalid_argument("one-electron derivatives cover l <= 2");
     for (int ish = 0; ish < ns; ish++)
         for (int jsh = 0; jsh < ns; jsh++) {
             const Shell1 &A = in.shells[ish], &B = in.shells[jsh];

@@ -43,7 +43,7 @@ void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, 
     std::vector<Shell1Dev> sh(ns);
     for (int s = 0; s < ns; s++) {
         const Shell1 &x = in.shells[s];
-        if (x.l < 0 || x.l > 2) throw std::invalid_argument("one-electron integrals on the GPU cover l <= 2");
+        if (x.l < 0 || x.l > 4) throw std::invalid_argument("one-electron integrals on the GPU cover l <= 4");
         sh[s] = Shell1Dev{x.l, x.nprim, x.pexp, x.pcoef, x.ao, 0, {x.r[0], x.r[1], x.r[2]}};
     }
     std::vector<double4> nuc(in.charges.size());
@@ -51,9 +51,9 @@ void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, 
         nuc[c] = make_double4(in.centers[c][0], in.centers[c][1], in.centers[c][2], in.charges[c]);
     // ordered shell pairs, grouped by class
     std::vector<int2> pairs;
-    int first[3][3], count[3][3];
-    for (int li = 0; li < 3; li++)
-        for (int lj = 0; lj < 3; lj++) {
+    int first[5][5], count[5][5];
+    for (int li = 0; li < 5; li++)
+        for (int lj = 0; lj < 5; lj++) {
             first[li][lj] = (int)pairs.size();
             for (int i = 0; i < ns; i++)
                 if (sh[i].l == li)
@@ -65,8 +65,8 @@ void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, 
     DevBuf<double> denv(in.env);
     DevBuf<double4> dnuc(nuc);
     DevBuf<int2> dpairs(pairs);
-    for (int li = 0; li < 3; li++)
-        for (int lj = 0; lj < 3; lj++)
+    for (int li = 0; li < 5; li++)
+        for (int lj = 0; lj < 5; lj++)
             ck(launch_int1e(li, lj, dpairs.p + first[li][lj], count[li][lj], dsh.p, denv.p, dnuc.p, (int)nuc.size(),
                             in.nao, dS, dT, dV, st),
                "int1e launch");

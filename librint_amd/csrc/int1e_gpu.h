@@ -22,7 +22,7 @@ hipError_t launch_int1e(int li, int lj, const int2 *pairs, int npair, const Shel
                         const double4 *nuc, int nnuc, int nao, double *S, double *T, double *V, hipStream_t st);
 
 // Host driver: S, T, V (any may be null) into host buffers nao*nao, computed
-// on the current GPU.  Throws on failure; l > 2 is rejected.
+// on the current GPU.  Throws on failure; l > 4 is rejected.
 void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V);
 // Same into device buffers (nao*nao each, any may be null), on stream st.
 void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, hipStream_t st);

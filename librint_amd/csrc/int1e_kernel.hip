@@ -1,4 +1,4 @@
-// One-electron integrals S, T, V on the GPU (cartesian, l <= 2): semantics of
+// One-electron integrals S, T, V on the GPU (cartesian, l <= 4): semantics of
 // the reference's cint1e_{ovlp,kin,nuc}_cart (src/cint1e.rs:925-1025,
 // src/intor1.rs:168-218, src/g1e.rs:241-395): coefficients as stored in env
 // times common_fac_sp(l), point nuclei, V = sum_C -Z_C <i|1/|r-C||j>,
@@ -15,10 +15,12 @@
 
 namespace librint {
 
-__device__ __constant__ signed char C1E[3][6][3] = {
-    {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
-    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
-    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
+__device__ __constant__ signed char C1E[5][15][3] = {
+    {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+    {{3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
+    {{4, 0, 0}, {3, 1, 0}, {3, 0, 1}, {2, 2, 0}, {2, 1, 1}, {2, 0, 2}, {1, 3, 0}, {1, 2, 1}, {1, 1, 2}, {1, 0, 3}, {0, 4, 0}, {0, 3, 1}, {0, 2, 2}, {0, 1, 3}, {0, 0, 4}}};
 
 __device__ __forceinline__ double fac_sp_dev(int l)
 {
@@ -160,6 +162,8 @@ hipError_t launch_int1e(int li, int lj, const int2 *pairs, int npair, const Shel
         return hipGetLastError();                                                                                      \
     }
     X(0, 0) X(0, 1) X(0, 2) X(1, 0) X(1, 1) X(1, 2) X(2, 0) X(2, 1) X(2, 2)
+    X(0, 3) X(1, 3) X(2, 3) X(3, 0) X(3, 1) X(3, 2) X(3, 3)
+    X(0, 4) X(1, 4) X(2, 4) X(3, 4) X(4, 0) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -21,6 +21,35 @@ struct DevBasis {
     const int *ao_off;
 };
 
+// One shell pair as the kernels read it: shells, primitive-pair records
+// [off, off + npp), AO offsets of the two shells, Rb and Rb - Ra.
+struct PairView {
+    int sa, sb, off, npp;
+    int aoa, aob;
+    double rb[3], rab[3];
+};
+
+__device__ __forceinline__ PairView load_pair(const DevBasis &B, int p)
+{
+    PairView v;
+    const int4 pi = B.pair_info[p];
+    v.sa = pi.x;
+    v.sb = pi.y;
+    v.off = pi.z;
+    v.npp = pi.w;
+    v.aoa = B.ao_off[v.sa];
+    v.aob = B.ao_off[v.sb];
+    const double4 g0 = B.pair_rb[p];
+    const double4 g1 = B.pair_rab[p];
+    v.rb[0] = g0.x;
+    v.rb[1] = g0.y;
+    v.rb[2] = g0.z;
+    v.rab[0] = g1.x;
+    v.rab[1] = g1.y;
+    v.rab[2] = g1.z;
+    return v;
+}
+
 // One wavefront of the Fock kernel: bra pair x kets [ket0, ket0 + nket) of
 // the pair list (all kets of one bin).
 struct JKTask {

@@ -31,33 +31,6 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-struct PairView {
-    int sa, sb, off, npp;
-    int aoa, aob;
-    double rb[3], rab[3];
-};
-
-__device__ __forceinline__ PairView load_pair(const DevBasis &B, int p)
-{
-    PairView v;
-    const int4 pi = B.pair_info[p];
-    v.sa = pi.x;
-    v.sb = pi.y;
-    v.off = pi.z;
-    v.npp = pi.w;
-    v.aoa = B.ao_off[v.sa];
-    v.aob = B.ao_off[v.sb];
-    const double4 g0 = B.pair_rb[p];
-    const double4 g1 = B.pair_rab[p];
-    v.rb[0] = g0.x;
-    v.rb[1] = g0.y;
-    v.rb[2] = g0.z;
-    v.rab[0] = g1.x;
-    v.rab[1] = g1.y;
-    v.rab[2] = g1.z;
-    return v;
-}
-
 template <int LA, int LB, int LC, int LD>
 __device__ __forceinline__ void contracted_fp64(double *eri, const DevBasis &B, const PairView &bra, const PairView &ket)
 {
@@ -731,6 +704,10 @@ __global__ __launch_bounds__(64) void eri_list_team_kernel(const ListTask *__res
             out[t.out] = sqrt(mm);
         }
     } else {
+        // entries related by a same-shell pair or by bra = ket map to the
+        // same 8 addresses and are held by different lanes: only the
+        // canonical one writes, so every image carries one value
+        const bool sab = bra.sa == bra.sb, scd = ket.sa == ket.sb, sbk = t.bra == t.ket;
 #pragma unroll
         for (int k = 0; k < S::NPL; k++) {
             const int f = lane + 64 * k;
@@ -738,6 +715,7 @@ __global__ __launch_bounds__(64) void eri_list_team_kernel(const ListTask *__res
             const int fa = f % C::NFA, t1 = f / C::NFA;
             const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
             const int fc = t2 % C::NFC, fd = t2 / C::NFC;
+            if ((sab && fa < fb) || (scd && fc < fd) || (sbk && fa * C::NFB + fb < fc * C::NFD + fd)) continue;
             const double v = acc[k];
             const size_t a = bra.aoa + fa, b = bra.aob + fb, c = ket.aoa + fc, d = ket.aob + fd;
             out[((a * n + b) * n + c) * n + d] = v;

@@ -312,6 +312,21 @@ def water_cluster(n_water, seed=WATER_SEED):
     return build(atoms, _even_tempered_basis(), "w%d_et" % n_water, normalized=False)
 
 
+def h2o_fg():
+    """H2O with f and g shells: STO-3G plus, on O, a d (zeta 0.8), a
+    two-primitive f (1.3, 0.45) and a g (1.1); on H a p (0.75) and a d
+    (1.0).  Parity molecule for the l = 3, 4 paths (not a published basis)."""
+    atoms = [(8, _ang(WATER_INTERNAL[0])), (1, _ang(WATER_INTERNAL[1])), (1, _ang(WATER_INTERNAL[2]))]
+    basis = {8: STO3G[8] + [(2, [0.8], [1.0]), (3, [1.3, 0.45], [0.6, 0.5]), (4, [1.1], [1.0])],
+             1: STO3G[1] + [(1, [0.75], [1.0]), (2, [1.0], [1.0])]}
+    # STO-3G coefficients are already normalised; normalise the added shells
+    norm = {}
+    for Z, shells in basis.items():
+        norm[Z] = [sh if k < len(STO3G[Z]) else (sh[0], sh[1], normalize_contraction(sh[0], sh[1], sh[2]))
+                   for k, sh in enumerate(shells)]
+    return build(atoms, norm, "h2o_fg")
+
+
 def h2o_631gstar_like():
     """Small d-shell test molecule: H2O with the carbon 6-31G* d shell on O
     (parameters only; used for d-shell parity, not a published basis)."""

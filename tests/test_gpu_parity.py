@@ -162,9 +162,9 @@ def test_failures_are_reported(gpu):
         scf.scf(m, 1, 1e-12)
     from librint_amd import mol
     atoms = [(8, (0.0, 0.0, 0.0)), (1, (0.0, 1.4, 1.1)), (1, (0.0, -1.4, 1.1))]
-    f_shell = mol.build(atoms, {8: mol.STO3G[8] + [(3, [0.8], [1.0])], 1: mol.STO3G[1]}, "h2o_f")
+    h_shell = mol.build(atoms, {8: mol.STO3G[8] + [(5, [0.8], [1.0])], 1: mol.STO3G[1]}, "h2o_h")
     with pytest.raises(RuntimeError):
-        scf.int2e(f_shell)  # l = 3 is outside the GPU kernels
+        scf.int2e(h_shell)  # l = 5 is outside the GPU kernels
 
 
 def test_fock_device_buffers(gpu):
@@ -228,6 +228,9 @@ def test_tensor_d_shells(gpu):
     R = scf.int2e(m)
     Rr = refcpu.int2e(m)
     assert np.abs(R - Rr).max() < 1e-12
+    # every image of a quartet carries the same value (one writer per image)
+    for perm in [(1, 0, 2, 3), (2, 3, 0, 1), (3, 2, 1, 0)]:
+        assert np.abs(R - R.transpose(perm)).max() == 0.0
 
 
 def _core_density(m):

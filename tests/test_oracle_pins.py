@@ -143,3 +143,29 @@ def test_planted_mistake_is_caught():
     R = refcpu.int2e(m)
     wrong = np.einsum("ijkl,lk->ij", R, P) + 0.5 * np.einsum("ijkl,jk->il", R, P)
     assert np.abs(refcpu.fock2e(m, P) - wrong).max() > 1e-3
+
+
+def test_eri_vs_textbook_f_g_shells():
+    """l = 3, 4 quartets of the oracle against the McMurchie-Davidson
+    restatement (f on O contracted from two primitives, g uncontracted)."""
+    from librint_amd import mol
+    m = mol.h2o_fg()
+    # shells: O 0 s, 1 s, 2 p, 3 d, 4 f, 5 g; H 6 s, 7 p, 8 d; H 9 s, 10 p, 11 d
+    for q in [(5, 0, 0, 0), (4, 4, 0, 0), (5, 2, 3, 6), (4, 3, 4, 9), (5, 5, 0, 1), (4, 8, 2, 11),
+              (5, 4, 7, 0), (0, 5, 6, 4), (10, 4, 5, 9)]:
+        a = tb.eri_block(m, q)
+        assert np.abs(a - refcpu.int2e_shell(m, q)).max() < 1e-14 * max(1.0, np.abs(a).max()), q
+
+
+@pytest.mark.parametrize("typ", [0, 1, 2])
+def test_int1e_vs_textbook_f_g_shells(typ):
+    """Every one-electron block of the f/g molecule: oracle vs the
+    McMurchie-Davidson S, T (ket Laplacian), V (Hermite R integrals)."""
+    from librint_amd import mol
+    m = mol.h2o_fg()
+    R = refcpu.int1e(m, typ)
+    off = np.cumsum([0] + [(int(b[1]) + 1) * (int(b[1]) + 2) // 2 for b in m._bas])
+    for i in range(m.nbas):
+        for j in range(m.nbas):
+            blk = tb.int1e_block(m, i, j, typ)
+            assert np.abs(blk - R[off[i]:off[i + 1], off[j]:off[j + 1]]).max() < 1e-13, (typ, i, j)

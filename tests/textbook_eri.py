@@ -128,3 +128,52 @@ def eri_block(mol, shls):
                                                             s += e1 * sgn * e2 * R[t + tau, u + nu, v + phi]
                                     out[ia, ib, ic, id_] += pref * s
     return out
+
+
+def _ovlp_1d(i, j, a, b, XAB):
+    """1D overlap of x_A^i x_B^j exp(-a x_A^2 - b x_B^2), E^{ij}_0 sqrt(pi/p)."""
+    if i < 0 or j < 0:
+        return 0.0
+    return hermite_E(i, j, a, b, XAB)[i, j, 0] * math.sqrt(math.pi / (a + b))
+
+
+def int1e_block(mol, i, j, typ):
+    """One-electron block (nfi, nfj) by McMurchie-Davidson: typ 0 overlap,
+    1 kinetic -1/2 <i|lap|j>, 2 nuclear sum_C -Z_C <i|1/r_C|j>."""
+    li, A, ea, ca = shell(mol, i)
+    lj, B, eb, cb = shell(mol, j)
+    fi, fj = cart(li), cart(lj)
+    out = np.zeros((len(fi), len(fj)))
+    atoms = [(abs(int(a[0])), mol._env[a[1]:a[1] + 3]) for a in mol._atm]
+    for a, c1 in zip(ea, ca):
+        for b, c2 in zip(eb, cb):
+            p = a + b
+            P = (a * A + b * B) / p
+            E = [hermite_E(li, lj + 2, a, b, A[x] - B[x]) for x in range(3)]
+            for ia, pa in enumerate(fi):
+                for jb, pb in enumerate(fj):
+                    if typ == 0:
+                        v = np.prod([_ovlp_1d(pa[x], pb[x], a, b, A[x] - B[x]) for x in range(3)])
+                    elif typ == 1:
+                        S = [_ovlp_1d(pa[x], pb[x], a, b, A[x] - B[x]) for x in range(3)]
+                        v = 0.0
+                        for x in range(3):
+                            jx = pb[x]
+                            tx = (-0.5 * jx * (jx - 1) * _ovlp_1d(pa[x], jx - 2, a, b, A[x] - B[x])
+                                  + b * (2 * jx + 1) * S[x]
+                                  - 2.0 * b * b * _ovlp_1d(pa[x], jx + 2, a, b, A[x] - B[x]))
+                            v += tx * S[(x + 1) % 3] * S[(x + 2) % 3]
+                    else:
+                        v = 0.0
+                        L = li + lj
+                        for Z, C in atoms:
+                            R = hermite_R(L, p, P - C)
+                            acc = 0.0
+                            for t in range(pa[0] + pb[0] + 1):
+                                for u in range(pa[1] + pb[1] + 1):
+                                    for w in range(pa[2] + pb[2] + 1):
+                                        acc += E[0][pa[0], pb[0], t] * E[1][pa[1], pb[1], u] * \
+                                            E[2][pa[2], pb[2], w] * R[t, u, w]
+                            v -= Z * 2.0 * math.pi / p * acc
+                    out[ia, jb] += c1 * c2 * v
+    return out
