@@ -22,7 +22,8 @@ struct Int1eInput {
 Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env);
 double nuclear_repulsion(const Int1eInput &in);
 
-// Derivatives of E1 = sum_ab W_ab O_ab (W symmetric, nao*nao; O = S (typ 0),
+// On the GPU (int1e_gpu.cpp): derivatives of E1 = sum_ab W_ab O_ab (W
+// symmetric host matrix nao*nao; O = S (typ 0),
 // T (1) or V (2)) with respect to every primitive exponent and contraction
 // coefficient (added into denv[env index], entries below env_l) and to the
 // nuclear coordinates (added into gatom[3*natm]; for V this includes the

@@ -1,6 +1,6 @@
 // Host driver of the one-electron kernels (int1e_kernel.hip): stages shells, env and
 // point charges on the current GPU, launches one kernel per shell-pair class
-// and copies the matrices back.
+// and copies the matrices (or the derivative sums) back.
 #include "int1e_gpu.h"
 
 #include <stdexcept>
@@ -34,44 +34,105 @@ struct DevBuf {
     DevBuf &operator=(const DevBuf &) = delete;
 };
 
+// Shells (atom index in pad), point charges and ordered shell pairs grouped
+// by (li, lj) for lmax.
+struct Staged {
+    std::vector<Shell1Dev> sh;
+    std::vector<double4> nuc;
+    std::vector<int2> pairs;
+    int first[5][5] = {}, count[5][5] = {};
+};
+
+Staged stage(const Int1eInput &in, int lmax, const char *what)
+{
+    Staged g;
+    const int ns = (int)in.shells.size();
+    g.sh.resize(ns);
+    for (int s = 0; s < ns; s++) {
+        const Shell1 &x = in.shells[s];
+        if (x.l < 0 || x.l > lmax) throw std::invalid_argument(what);
+        g.sh[s] = Shell1Dev{x.l, x.nprim, x.pexp, x.pcoef, x.ao, x.atom, {x.r[0], x.r[1], x.r[2]}};
+    }
+    g.nuc.resize(in.charges.size());
+    for (size_t c = 0; c < g.nuc.size(); c++)
+        g.nuc[c] = make_double4(in.centers[c][0], in.centers[c][1], in.centers[c][2], in.charges[c]);
+    for (int li = 0; li <= lmax; li++)
+        for (int lj = 0; lj <= lmax; lj++) {
+            g.first[li][lj] = (int)g.pairs.size();
+            for (int i = 0; i < ns; i++)
+                if (g.sh[i].l == li)
+                    for (int j = 0; j < ns; j++)
+                        if (g.sh[j].l == lj) g.pairs.push_back(make_int2(i, j));
+            g.count[li][lj] = (int)g.pairs.size() - g.first[li][lj];
+        }
+    return g;
+}
+
 }  // namespace
 
 void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, hipStream_t st)
 {
-    const int ns = (int)in.shells.size();
     if (!dS && !dT && !dV) return;
-    std::vector<Shell1Dev> sh(ns);
-    for (int s = 0; s < ns; s++) {
-        const Shell1 &x = in.shells[s];
-        if (x.l < 0 || x.l > 4) throw std::invalid_argument("one-electron integrals on the GPU cover l <= 4");
-        sh[s] = Shell1Dev{x.l, x.nprim, x.pexp, x.pcoef, x.ao, 0, {x.r[0], x.r[1], x.r[2]}};
-    }
-    std::vector<double4> nuc(in.charges.size());
-    for (size_t c = 0; c < nuc.size(); c++)
-        nuc[c] = make_double4(in.centers[c][0], in.centers[c][1], in.centers[c][2], in.charges[c]);
-    // ordered shell pairs, grouped by class
-    std::vector<int2> pairs;
-    int first[5][5], count[5][5];
-    for (int li = 0; li < 5; li++)
-        for (int lj = 0; lj < 5; lj++) {
-            first[li][lj] = (int)pairs.size();
-            for (int i = 0; i < ns; i++)
-                if (sh[i].l == li)
-                    for (int j = 0; j < ns; j++)
-                        if (sh[j].l == lj) pairs.push_back(make_int2(i, j));
-            count[li][lj] = (int)pairs.size() - first[li][lj];
-        }
-    DevBuf<Shell1Dev> dsh(sh);
+    const Staged g = stage(in, 4, "one-electron integrals on the GPU cover l <= 4");
+    DevBuf<Shell1Dev> dsh(g.sh);
     DevBuf<double> denv(in.env);
-    DevBuf<double4> dnuc(nuc);
-    DevBuf<int2> dpairs(pairs);
+    DevBuf<double4> dnuc(g.nuc);
+    DevBuf<int2> dpairs(g.pairs);
     for (int li = 0; li < 5; li++)
         for (int lj = 0; lj < 5; lj++)
-            ck(launch_int1e(li, lj, dpairs.p + first[li][lj], count[li][lj], dsh.p, denv.p, dnuc.p, (int)nuc.size(),
-                            in.nao, dS, dT, dV, st),
+            ck(launch_int1e(li, lj, dpairs.p + g.first[li][lj], g.count[li][lj], dsh.p, denv.p, dnuc.p,
+                            (int)g.nuc.size(), in.nao, dS, dT, dV, st),
                "int1e launch");
     // the staging buffers are freed on return: finish the launches first
     ck(hipStreamSynchronize(st), "hipStreamSynchronize");
+}
+
+// One derivative pass on the GPU: W-weighted sums into gatom/denv (added to
+// the host arrays), or the dS matrix (typ 0, W null).
+static void deriv_gpu(const Int1eInput &in, int typ, const double *W, double *gatom, double *denv, size_t env_l,
+                      double *dSout, size_t s1, size_t L)
+{
+    const Staged g = stage(in, 2, "one-electron derivatives cover l <= 2");
+    const size_t nn = (size_t)in.nao * in.nao;
+    const size_t na = in.charges.size();
+    DevBuf<Shell1Dev> dsh(g.sh);
+    DevBuf<double> denv_in(in.env);
+    DevBuf<double4> dnuc(g.nuc);
+    DevBuf<int2> dpairs(g.pairs);
+    DevBuf<double> dW(W ? nn : 0), dga(gatom ? 3 * na : 0), dde(denv ? env_l : 0), ddS(dSout ? nn * L : 0);
+    if (W) ck(hipMemcpy(dW.p, W, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    if (gatom) ck(hipMemset(dga.p, 0, sizeof(double) * 3 * na), "hipMemset");
+    if (denv && env_l) ck(hipMemset(dde.p, 0, sizeof(double) * env_l), "hipMemset");
+    if (dSout && nn * L) ck(hipMemset(ddS.p, 0, sizeof(double) * nn * L), "hipMemset");
+    for (int li = 0; li < 3; li++)
+        for (int lj = 0; lj < 3; lj++)
+            ck(launch_int1e_deriv(li, lj, dpairs.p + g.first[li][lj], g.count[li][lj], dsh.p, denv_in.p, dnuc.p,
+                                  (int)g.nuc.size(), in.nao, typ, dW.p, dga.p, dde.p, (long long)env_l, ddS.p,
+                                  (long long)s1, (long long)L, 0),
+               "one-electron derivative launch");
+    if (gatom) {
+        std::vector<double> h(3 * na);
+        ck(hipMemcpy(h.data(), dga.p, sizeof(double) * 3 * na, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        for (size_t k = 0; k < 3 * na; k++) gatom[k] += h[k];
+    }
+    if (denv && env_l) {
+        std::vector<double> h(env_l);
+        ck(hipMemcpy(h.data(), dde.p, sizeof(double) * env_l, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        for (size_t k = 0; k < env_l; k++) denv[k] += h[k];
+    }
+    if (dSout && nn * L) ck(hipMemcpy(dSout, ddS.p, sizeof(double) * nn * L, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+void int1e_grad(const Int1eInput &in, int typ, const double *W, double *gatom, double *denv, size_t env_l)
+{
+    if (typ < 0 || typ > 2) throw std::invalid_argument("one-electron derivative type must be 0, 1 or 2");
+    if (!W) throw std::invalid_argument("null weight matrix");
+    deriv_gpu(in, typ, W, gatom, denv, env_l, nullptr, 0, 0);
+}
+
+void overlap_param_derivs(const Int1eInput &in, size_t s1, size_t L, double *dS)
+{
+    deriv_gpu(in, 0, nullptr, nullptr, nullptr, 0, dS, s1, L);
 }
 
 void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V)

@@ -21,6 +21,15 @@ struct Shell1Dev {
 hipError_t launch_int1e(int li, int lj, const int2 *pairs, int npair, const Shell1Dev *sh, const double *env,
                         const double4 *nuc, int nnuc, int nao, double *S, double *T, double *V, hipStream_t st);
 
+// Bra derivatives of the primitive integrals of the ordered pairs of class
+// (li, lj) <= 2, weighted by W (W-weighted sums into gatom[3*natm] and
+// denv[env index < env_l], any may be null) or, for typ 0 with dS non-null,
+// as the matrix dS[(a*nao + b)*L + (e - s1)] for env entries s1 <= e < s1+L.
+// shells' pad field holds the atom index.
+hipError_t launch_int1e_deriv(int li, int lj, const int2 *pairs, int npair, const Shell1Dev *sh, const double *env,
+                              const double4 *nuc, int nnuc, int nao, int typ, const double *W, double *gatom,
+                              double *denv, long long env_l, double *dS, long long s1, long long L, hipStream_t st);
+
 // Host driver: S, T, V (any may be null) into host buffers nao*nao, computed
 // on the current GPU.  Throws on failure; l > 4 is rejected.
 void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V);

@@ -149,6 +149,234 @@ __global__ __launch_bounds__(128) void int1e_kernel(const int2 *__restrict__ pai
         }
 }
 
+// ------------------------------------------------------------ derivatives --
+// Bra derivatives of the primitive integrals of one ordered shell pair (the
+// semantics of the reference's dscf.rs one-electron parts: src/dscf.rs:96-150
+// for dS, :320-336 for dHcore).  With W and O symmetric, the ket derivative
+// of O_ab is the bra derivative of O_ba, so sum_ab W_ab dO_ab = 2 sum_ab W_ab
+// d_bra O_ab; a point charge C enters V^C only through A - C and B - C, so
+// d/dC V^C = -(d/dA + d/dB) V^C.  Per primitive x_A^a exp(-alpha r_A^2):
+//   d/dA_x = 2 alpha G(a + 1_x) - a_x G(a - 1_x),  d/dalpha = -sum_x G(a + 2_x),
+//   d/dc = G / c.
+// One thread per ordered shell pair; the 1D tables hold bra powers up to
+// LI + 2.
+
+// value of the primitive integral with bra powers ia, ket powers jb
+template <int LI, int LJ>
+struct DerivTables {
+    static constexpr int IM = LI + 3, JS = LJ + 3, JV = LJ + 1, NR = (LI + 2 + LJ) / 2 + 1;
+    double s[3][IM][JS];   // 1D overlaps (S, T)
+    double t[3][IM][JV];   // 1D kinetic factors (T)
+    double I[NR][3][IM][JV];  // 1D Rys integrals times weights folded later (V)
+    double w[NR];
+};
+
+template <int LI, int LJ>
+__device__ __forceinline__ double deriv_val(const DerivTables<LI, LJ> &T, int typ, const int *ia, const int *jb)
+{
+    if (typ == 0) return T.s[0][ia[0]][jb[0]] * T.s[1][ia[1]][jb[1]] * T.s[2][ia[2]][jb[2]];
+    if (typ == 1) {
+        const double sx = T.s[0][ia[0]][jb[0]], sy = T.s[1][ia[1]][jb[1]], sz = T.s[2][ia[2]][jb[2]];
+        return T.t[0][ia[0]][jb[0]] * sy * sz + sx * T.t[1][ia[1]][jb[1]] * sz + sx * sy * T.t[2][ia[2]][jb[2]];
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < DerivTables<LI, LJ>::NR; r++)
+        acc += T.w[r] * T.I[r][0][ia[0]][jb[0]] * T.I[r][1][ia[1]][jb[1]] * T.I[r][2][ia[2]][jb[2]];
+    return acc;
+}
+
+// W-weighted sums of one charge (typ 2) or of S/T over the components of
+// the pair: adds exponent / coefficient derivatives of bra primitive ip and
+// the bra-centre gradient; dS mode writes the matrix entries instead.
+template <int LI, int LJ>
+__device__ __forceinline__ void deriv_accumulate(const DerivTables<LI, LJ> &T, int typ, double pref, double alpha,
+                                                 double ca, const Shell1Dev &A, const Shell1Dev &B, int ip,
+                                                 const double *W, int nao, double &dex, double &dco, double gA[3],
+                                                 double *dS, long long s1, long long L)
+{
+    constexpr int NFI = (LI + 1) * (LI + 2) / 2, NFJ = (LJ + 1) * (LJ + 2) / 2;
+    const size_t n = (size_t)nao;
+    for (int fj = 0; fj < NFJ; fj++)
+        for (int fi = 0; fi < NFI; fi++) {
+            const int ia[3] = {C1E[LI][fi][0], C1E[LI][fi][1], C1E[LI][fi][2]};
+            const int jb[3] = {C1E[LJ][fj][0], C1E[LJ][fj][1], C1E[LJ][fj][2]};
+            const double v0 = pref * deriv_val<LI, LJ>(T, typ, ia, jb);
+            double de = 0.0, dA[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                int up[3] = {ia[0], ia[1], ia[2]};
+                up[d] += 2;
+                de -= pref * deriv_val<LI, LJ>(T, typ, up, jb);
+                int p1[3] = {ia[0], ia[1], ia[2]};
+                p1[d] += 1;
+                double g = 2.0 * alpha * pref * deriv_val<LI, LJ>(T, typ, p1, jb);
+                if (ia[d] > 0) {
+                    int m1[3] = {ia[0], ia[1], ia[2]};
+                    m1[d] -= 1;
+                    g -= ia[d] * pref * deriv_val<LI, LJ>(T, typ, m1, jb);
+                }
+                dA[d] = g;
+            }
+            const size_t a = A.ao + fi, b = B.ao + fj;
+            if (W) {
+                const double w2 = 2.0 * W[a * n + b];
+                dex += w2 * ca * de;
+                dco += w2 * v0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) gA[d] += w2 * ca * dA[d];
+            }
+            if (dS) {
+                const long long ie = A.pexp + ip, ic = A.pcoef + ip;
+                if (ie >= s1 && ie < s1 + L) {
+                    atomicAdd(&dS[(a * n + b) * L + (ie - s1)], ca * de);
+                    atomicAdd(&dS[(b * n + a) * L + (ie - s1)], ca * de);
+                }
+                if (ic >= s1 && ic < s1 + L) {
+                    atomicAdd(&dS[(a * n + b) * L + (ic - s1)], v0);
+                    atomicAdd(&dS[(b * n + a) * L + (ic - s1)], v0);
+                }
+            }
+        }
+}
+
+template <int LI, int LJ>
+__global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict__ pairs, int npair,
+                                                         const Shell1Dev *__restrict__ sh,
+                                                         const double *__restrict__ env,
+                                                         const double4 *__restrict__ nuc, int nnuc, int nao, int typ,
+                                                         const double *__restrict__ W, double *__restrict__ gatom,
+                                                         double *__restrict__ denv, long long env_l,
+                                                         double *__restrict__ dS, long long s1, long long L)
+{
+    using TT = DerivTables<LI, LJ>;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= npair) return;
+    const int2 pr = pairs[t];
+    const Shell1Dev A = sh[pr.x], B = sh[pr.y];  // A.pad = atom of A
+    double AB[3], AB2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        AB[d] = A.r[d] - B.r[d];
+        AB2 += AB[d] * AB[d];
+    }
+    const double cf = fac_sp_dev(LI) * fac_sp_dev(LJ);
+    TT T;
+    double gAt[3] = {0.0, 0.0, 0.0};
+    for (int ip = 0; ip < A.nprim; ip++) {
+        const double a = env[A.pexp + ip], ca = env[A.pcoef + ip];
+        double dex = 0.0, dco = 0.0;
+        for (int jp = 0; jp < B.nprim; jp++) {
+            const double b = env[B.pexp + jp], cb = env[B.pcoef + jp];
+            const double p = a + b, h = 0.5 / p;
+            double P[3];
+#pragma unroll
+            for (int d = 0; d < 3; d++) P[d] = (a * A.r[d] + b * B.r[d]) / p;
+            const double pref0 = cb * cf * exp(-a * b / p * AB2);  // bra coefficient left out
+            if (typ <= 1) {
+                const double s00 = sqrt(M_PI / p);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    const double PA = P[d] - A.r[d], PB = P[d] - B.r[d];
+#pragma unroll
+                    for (int i = 0; i < TT::IM; i++)
+#pragma unroll
+                        for (int j = 0; j < TT::JS; j++) {
+                            double v;
+                            if (i == 0 && j == 0) v = s00;
+                            else if (i > 0) {
+                                v = PA * T.s[d][i - 1][j];
+                                if (i > 1) v += (i - 1) * h * T.s[d][i - 2][j];
+                                if (j > 0) v += j * h * T.s[d][i - 1][j - 1];
+                            } else {
+                                v = PB * T.s[d][i][j - 1];
+                                if (j > 1) v += (j - 1) * h * T.s[d][i][j - 2];
+                            }
+                            T.s[d][i][j] = v;
+                        }
+                    if (typ == 1) {
+#pragma unroll
+                        for (int i = 0; i < TT::IM; i++)
+#pragma unroll
+                            for (int j = 0; j < TT::JV; j++) {
+                                double tt = -2.0 * b * (2 * j + 1) * T.s[d][i][j] + 4.0 * b * b * T.s[d][i][j + 2];
+                                if (j >= 2) tt += j * (j - 1) * T.s[d][i][j - 2];
+                                T.t[d][i][j] = -0.5 * tt;
+                            }
+                    }
+                }
+                deriv_accumulate<LI, LJ>(T, typ, pref0, a, ca, A, B, ip, W, nao, dex, dco, gAt, dS, s1, L);
+            } else {
+                for (int c = 0; c < nnuc; c++) {
+                    const double4 C = nuc[c];
+                    const double PC[3] = {P[0] - C.x, P[1] - C.y, P[2] - C.z};
+                    const double PC2 = PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2];
+                    double rv[TT::NR];
+                    rys_nodes<TT::NR>(p * PC2, rv, T.w);
+                    constexpr int LT = LI + 2 + LJ;
+#pragma unroll
+                    for (int r = 0; r < TT::NR; r++) {
+                        const double b10 = (1.0 - rv[r]) * h;
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            const double c00 = (P[d] - A.r[d]) - rv[r] * PC[d];
+                            double H[LT + 1][LJ + 1];
+                            H[0][0] = 1.0;
+                            if (LT > 0) H[1][0] = c00;
+#pragma unroll
+                            for (int i = 1; i < LT; i++) H[i + 1][0] = c00 * H[i][0] + i * b10 * H[i - 1][0];
+#pragma unroll
+                            for (int j = 1; j <= LJ; j++)
+#pragma unroll
+                                for (int i = 0; i <= LT - j; i++) H[i][j] = H[i + 1][j - 1] + AB[d] * H[i][j - 1];
+#pragma unroll
+                            for (int i = 0; i < TT::IM; i++)
+#pragma unroll
+                                for (int j = 0; j < TT::JV; j++) T.I[r][d][i][j] = H[i][j];
+                        }
+                    }
+                    double gA[3] = {0.0, 0.0, 0.0};
+                    const double pref = -C.w * 2.0 * M_PI / p * pref0;
+                    deriv_accumulate<LI, LJ>(T, 2, pref, a, ca, A, B, ip, W, nao, dex, dco, gA, dS, s1, L);
+                    if (gatom) {
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            gAt[d] += gA[d];
+                            atomicAdd(&gatom[3 * c + d], -gA[d]);
+                        }
+                    }
+                }
+            }
+        }
+        if (denv) {
+            if (A.pexp + ip < env_l) atomicAdd(&denv[A.pexp + ip], dex);
+            if (A.pcoef + ip < env_l) atomicAdd(&denv[A.pcoef + ip], dco);
+        }
+    }
+    if (gatom) {
+#pragma unroll
+        for (int d = 0; d < 3; d++) atomicAdd(&gatom[3 * A.pad + d], gAt[d]);
+    }
+}
+
+hipError_t launch_int1e_deriv(int li, int lj, const int2 *pairs, int npair, const Shell1Dev *sh, const double *env,
+                              const double4 *nuc, int nnuc, int nao, int typ, const double *W, double *gatom,
+                              double *denv, long long env_l, double *dS, long long s1, long long L, hipStream_t st)
+{
+    if (npair <= 0) return hipSuccess;
+    if (typ < 0 || typ > 2 || (dS && typ != 0)) return hipErrorInvalidValue;
+    const dim3 grid((npair + 63) / 64);
+#define X(a, b)                                                                                                        \
+    if (li == a && lj == b) {                                                                                          \
+        hipLaunchKernelGGL((int1e_deriv_kernel<a, b>), grid, dim3(64), 0, st, pairs, npair, sh, env, nuc, nnuc, nao,   \
+                           typ, W, gatom, denv, env_l, dS, s1, L);                                                     \
+        return hipGetLastError();                                                                                      \
+    }
+    X(0, 0) X(0, 1) X(0, 2) X(1, 0) X(1, 1) X(1, 2) X(2, 0) X(2, 1) X(2, 2)
+#undef X
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_int1e(int li, int lj, const int2 *pairs, int npair, const Shell1Dev *sh, const double *env,
                         const double4 *nuc, int nnuc, int nao, double *S, double *T, double *V, hipStream_t st)
 {

@@ -503,16 +503,9 @@ __device__ __forceinline__ void team_chain(double *g, int c, const PrimPair &p, 
     const double pqx = p.pbx - q.pbx + rbd[0], pqy = p.pby - q.pby + rbd[1], pqz = p.pbz - q.pbz + rbd[2];
     const double rr = pqx * pqx + pqy * pqy + pqz * pqz;
     const double fac1 = p.K * q.K * rsq;
-    double v[C::NROOTS], w[C::NROOTS];
-    rys_nodes<C::NROOTS>(a0 * rr, v, w);
     const int r = c / 3, ax = c - 3 * (c / 3);
-    double t2 = v[0], wr = w[0];
-#pragma unroll
-    for (int k = 1; k < C::NROOTS; k++)
-        if (r == k) {
-            t2 = v[k];
-            wr = w[k];
-        }
+    double t2, wr;
+    rys_one<C::NROOTS>(a0 * rr, r, t2, wr);  // only this lane's root
     const double tr = t2 * rs;
     const double b00 = 0.5 * tr;
     const double b10 = p.h * (1.0 - akl * tr);

@@ -29,6 +29,16 @@ def ref_truth(name, what):
     return np.loadtxt(os.path.join(REF, "truth", name, what + ".txt"))
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_device_first():
+    """torch carries its own HIP runtime, which must bring the device up
+    before the library's runtime does; any test may import the library, so
+    on a GPU machine torch's device is initialised before the first test."""
+    import torch
+    if torch.cuda.is_available():
+        torch.zeros(1, device="cuda:0")
+
+
 @pytest.fixture(scope="session")
 def gpu():
     import torch

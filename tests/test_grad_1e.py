@@ -1,7 +1,7 @@
-"""Host one-electron basis-parameter derivatives (no GPU needed):
-dHcore_c (reference src/dscf.rs:320-336) and dS_u (src/dscf.rs:96-150)
-against central finite differences of the oracle's one-electron integrals
-(an independent restatement), and the env split rule (src/utils.rs:152-175)."""
+"""One-electron basis-parameter derivatives (GPU kernels): dHcore_c
+(reference src/dscf.rs:320-336) and dS_u (src/dscf.rs:96-150) against
+central finite differences of the oracle's one-electron integrals (an
+independent restatement), and the env split rule (src/utils.rs:152-175)."""
 import numpy as np
 import pytest
 
@@ -23,8 +23,9 @@ def test_split_rule():
         assert s1 <= b[6] and b[6] + b[2] * b[3] <= s2
 
 
+@pytest.mark.gpu
 @pytest.mark.parametrize("k", range(3))
-def test_dhcore_and_dsu_vs_fd(k):
+def test_dhcore_and_dsu_vs_fd(gpu, k):
     from librint_amd import dscf
     from oracle import refcpu
     m = _mols()[k]
@@ -54,7 +55,8 @@ def test_dhcore_and_dsu_vs_fd(k):
         np.testing.assert_allclose(dSu[j - s1], dSu[j - s1].T, atol=1e-15)
 
 
-def test_dhcore_planted_mistake_is_caught():
+@pytest.mark.gpu
+def test_dhcore_planted_mistake_is_caught(gpu):
     """A dropped exponent term (the -a_x G(a-1) / +2 shift pair) would show:
     the FD check distinguishes exponent derivatives from half of themselves."""
     from librint_amd import dscf
