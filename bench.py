@@ -153,6 +153,9 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--tau-mix", type=float, default=1e-6)
     ap.add_argument("--tau-skip", type=float, default=0.0)
+    ap.add_argument("--tau-mix-compare", type=float, default=1e-5,
+                    help="FP64 runs also time the mixed-precision build at this threshold")
+    ap.add_argument("--no-mixed-compare", action="store_true")
     ap.add_argument("--grad", action="store_true",
                     help="time the analytic RHF gradient (adjoint ERI kernel) instead of the Fock build")
     ap.add_argument("--copy-mb", type=float, default=0.0,
@@ -263,6 +266,27 @@ def main():
                                      "ms": jk_ms, "flops": st["flops"]}},
         "classes": classes,
     }
+    if err is None and not args.no_mixed_compare:
+        # the same build with FP32 recurrences for the quartets whose
+        # Q_ab Q_cd max|D| < tau_mix, timed the same way, against this FP64 G
+        G64 = ctx.fock(P)
+        ctx.set_precision(1, args.tau_mix_compare, 0.0)
+        for _ in range(args.warmup):
+            ctx.fock_device(Pd, Gd)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.fock_device(Pd, Gd)
+        torch.cuda.synchronize()
+        ms_mixed = (time.perf_counter() - t0) / args.steps * 1e3
+        Gm = ctx.fock(P)
+        fc = ctx.flag_counts()
+        line["mixed_vs_fp64"] = {
+            "tau_mix": args.tau_mix_compare, "ms_per_step": ms_mixed, "fp64_ms_per_step": ms_per_step,
+            "speedup": ms_per_step / ms_mixed,
+            "energy_error_Eh": float(0.5 * np.sum(P * (Gm - G64))), "max_abs_dG": float(np.abs(Gm - G64).max()),
+            "quartets_fp64_fp32": fc["quartets"][:2]}
+        ctx.set_precision(0, 0.0, 0.0)
     if err is not None:
         line["metric"] = "RHF Fock-build s/iter + ERI quartets/s (%s fused J/K)" % ("mixed FP32/FP64" if mixed else "FP64, density screened")
         line["dtype"] = "f32+f64" if mixed else "f64"
