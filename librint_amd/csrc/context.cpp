@@ -41,6 +41,20 @@ static void cls_ls(int c, int &la, int &lb)
 
 static constexpr int NCLS = (HL_LMAX + 1) * (HL_LMAX + 2) / 2;
 
+// angular classes (la, lb, lc, ld) of a (bra class, ket class) keyed list
+template <typename M>
+static std::vector<std::array<int, 4>> list_classes(const M &by_cls)
+{
+    std::vector<std::array<int, 4>> out;
+    for (const auto &kv : by_cls) {
+        std::array<int, 4> c;
+        cls_ls(kv.first.first, c[0], c[1]);
+        cls_ls(kv.first.second, c[2], c[3]);
+        out.push_back(c);
+    }
+    return out;
+}
+
 static double common_fac_sp(int l)
 { // reference src/g1e.rs:776-782
     if (l == 0) return 0.282094791773878143;
@@ -256,6 +270,16 @@ void Context::compute_schwarz()
     double *dq = nullptr;
     check(hipMalloc(&dq, sizeof(double) * std::max(np, 1)), "hipMalloc");
     const DevBasis B = devb();
+    {
+        std::vector<std::array<int, 4>> cl;
+        for (int c = 0; c < NCLS; c++)
+            if (ranges[c].second) {
+                int la, lb;
+                cls_ls(c, la, lb);
+                cl.push_back({la, lb, la, lb});
+            }
+        hl_scratch_for(cl);
+    }
     for (int c = 0; c < NCLS; c++) {
         if (!ranges[c].second) continue;
         int la, lb;
@@ -295,14 +319,32 @@ void Context::compute_schwarz()
 double *Context::hl_scratch(const HLShape &S)
 {
     // one contracted block per workgroup; up to 8192 workgroups, at most
-    // 2^26 doubles (512 MiB)
+    // 2^26 doubles (512 MiB).  Callers size it for their largest class before
+    // queueing any launch: a reallocation here would free memory that an
+    // earlier launch may still be using.
     const size_t want = std::min<size_t>((size_t)1 << 26, (size_t)S.nf * 8192);
     if (want > nhlscr_) {
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         dev_free(d_hlscr_);
         check(hipMalloc(&d_hlscr_, sizeof(double) * want), "hipMalloc");
         nhlscr_ = want;
     }
     return d_hlscr_;
+}
+
+void Context::hl_scratch_for(const std::vector<std::array<int, 4>> &classes)
+{
+    const HLShape *big = nullptr;
+    HLShape best{};
+    for (const auto &c : classes)
+        if (hl_class(c[0], c[1], c[2], c[3])) {
+            const HLShape S = hl_shape(c[0], c[1], c[2], c[3]);
+            if (!big || S.nf > best.nf) {
+                best = S;
+                big = &best;
+            }
+        }
+    if (big) hl_scratch(best);
 }
 
 hipError_t Context::list_launch(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
@@ -654,6 +696,11 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     // but M (atomics) and, in the per-launch slab mode, their stream's slab
     // buffer.
     const int nl = (int)launches_.size();
+    {
+        std::vector<std::array<int, 4>> cl;
+        for (const ClassLaunch &L : launches_) cl.push_back({L.la, L.lb, L.lc, L.ld});
+        hl_scratch_for(cl);
+    }
     int ns = std::min(nstreams_, nl - 1);
     if (!jslab_whole_) ns = std::min(ns, slab_streams_);
     ns = std::max(ns, 1);
@@ -776,6 +823,7 @@ void Context::tensor_device(double *dR)
         }
     const DevBasis B = devb();
     std::vector<ListTask *> bufs;
+    hl_scratch_for(list_classes(by_cls));
     for (auto &kv : by_cls) {
         int la, lb, lc, ld;
         cls_ls(kv.first.first, la, lb);
@@ -849,6 +897,7 @@ size_t Context::shells_host(const int *quartets, int nq, double *out)
         check(hipMalloc(&dout, sizeof(double) * dev_total), "hipMalloc");
         const DevBasis B = devb();
         std::vector<ListTask *> bufs;
+        hl_scratch_for(list_classes(by_cls));
         for (auto &kv : by_cls) {
             int la, lb, lc, ld;
             cls_ls(kv.first.first, la, lb);

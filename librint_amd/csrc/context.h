@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -177,6 +178,7 @@ private:
     double *d_hlscr_ = nullptr;
     size_t nhlscr_ = 0;
     double *hl_scratch(const HLShape &S);
+    void hl_scratch_for(const std::vector<std::array<int, 4>> &classes);  // size for the largest
     hipError_t list_launch(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
                            double *out, int mode, hipStream_t st);
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
