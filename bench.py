@@ -153,8 +153,8 @@ def main():
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--tau-mix", type=float, default=1e-6)
     ap.add_argument("--tau-skip", type=float, default=0.0)
-    ap.add_argument("--tau-mix-compare", type=float, default=1e-5,
-                    help="FP64 runs also time the mixed-precision build at this threshold")
+    ap.add_argument("--tau-mix-compare", type=float, nargs="*", default=[1e-5, 1e-3],
+                    help="FP64 runs also time the mixed-precision build at these thresholds")
     ap.add_argument("--no-mixed-compare", action="store_true")
     ap.add_argument("--grad", action="store_true",
                     help="time the analytic RHF gradient (adjoint ERI kernel) instead of the Fock build")
@@ -257,7 +257,7 @@ def main():
                    "precision": "fp64", "quartets": int(quartets), "prim_quartets": int(st["prim_quartets"]),
                    "integrals": int(st["integrals"]), "candidates": int(st["candidates"]),
                    "launches": int(st["launches"]),
-                   "streams": int(os.environ.get("LIBRINT_STREAMS", "2"))},
+                   "streams": int(os.environ.get("LIBRINT_STREAMS", "4" if args.precision == "mixed" else "2"))},
         "roofline": {"bound": "mfma", "achieved": dom_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": dom_achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC.get((args.workload, args.precision, dom["kernel"])),
@@ -270,22 +270,24 @@ def main():
         # the same build with FP32 recurrences for the quartets whose
         # Q_ab Q_cd max|D| < tau_mix, timed the same way, against this FP64 G
         G64 = ctx.fock(P)
-        ctx.set_precision(1, args.tau_mix_compare, 0.0)
-        for _ in range(args.warmup):
-            ctx.fock_device(Pd, Gd)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            ctx.fock_device(Pd, Gd)
-        torch.cuda.synchronize()
-        ms_mixed = (time.perf_counter() - t0) / args.steps * 1e3
-        Gm = ctx.fock(P)
-        fc = ctx.flag_counts()
-        line["mixed_vs_fp64"] = {
-            "tau_mix": args.tau_mix_compare, "ms_per_step": ms_mixed, "fp64_ms_per_step": ms_per_step,
-            "speedup": ms_per_step / ms_mixed,
-            "energy_error_Eh": float(0.5 * np.sum(P * (Gm - G64))), "max_abs_dG": float(np.abs(Gm - G64).max()),
-            "quartets_fp64_fp32": fc["quartets"][:2]}
+        line["mixed_vs_fp64"] = []
+        for tau_mix in args.tau_mix_compare:
+            ctx.set_precision(1, tau_mix, 0.0)
+            for _ in range(args.warmup):
+                ctx.fock_device(Pd, Gd)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                ctx.fock_device(Pd, Gd)
+            torch.cuda.synchronize()
+            ms_mixed = (time.perf_counter() - t0) / args.steps * 1e3
+            Gm = ctx.fock(P)
+            fc = ctx.flag_counts()
+            line["mixed_vs_fp64"].append({
+                "tau_mix": tau_mix, "ms_per_step": ms_mixed, "fp64_ms_per_step": ms_per_step,
+                "speedup": ms_per_step / ms_mixed,
+                "energy_error_Eh": float(0.5 * np.sum(P * (Gm - G64))), "max_abs_dG": float(np.abs(Gm - G64).max()),
+                "quartets_fp64_fp32": fc["quartets"][:2]})
         ctx.set_precision(0, 0.0, 0.0)
     if err is not None:
         line["metric"] = "RHF Fock-build s/iter + ERI quartets/s (%s fused J/K)" % ("mixed FP32/FP64" if mixed else "FP64, density screened")

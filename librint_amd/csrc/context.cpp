@@ -587,7 +587,8 @@ void Context::build_tasks()
     // streams never share slab memory
     slab_streams_ = 1;
     if (!jslab_whole_ && largest > 0)
-        slab_streams_ = (int)std::max<size_t>(1, std::min<size_t>((size_t)nstreams_, JBUF_STREAMS / largest));
+        slab_streams_ = (int)std::max<size_t>(
+            1, std::min<size_t>((size_t)(nstreams_ > 0 ? nstreams_ : 4), JBUF_STREAMS / largest));
     jslot_ = largest;
     const size_t need = jslab_whole_ ? total : largest * slab_streams_;
     if (need > nJ_) {
@@ -696,12 +697,15 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     // but M (atomics) and, in the per-launch slab mode, their stream's slab
     // buffer.
     const int nl = (int)launches_.size();
+    // measured: two streams are best for the FP64 lists, four for the mixed
+    // ones (twice the launches, many of them small)
+    const int nstreams = nstreams_ > 0 ? nstreams_ : (precision_ == 1 ? 4 : 2);
     {
         std::vector<std::array<int, 4>> cl;
         for (const ClassLaunch &L : launches_) cl.push_back({L.la, L.lb, L.lc, L.ld});
         hl_scratch_for(cl);
     }
-    int ns = std::min(nstreams_, nl - 1);
+    int ns = std::min(nstreams, nl - 1);
     if (!jslab_whole_) ns = std::min(ns, slab_streams_);
     ns = std::max(ns, 1);
     if (ns > 1 && (int)side_.size() < ns - 1) {

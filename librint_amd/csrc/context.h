@@ -188,7 +188,7 @@ private:
     // caller's stream and nstreams_ - 1 extra streams (LIBRINT_STREAMS),
     // joined before the Coulomb reduction: small launches and launch tails
     // overlap
-    int nstreams_ = 2;
+    int nstreams_ = 0;      // 0: 2 for FP64 lists, 4 for mixed ones
     int dominant_ = 0;      // launch with the most flops: runs first, alone
     int slab_streams_ = 1;  // per-launch slab mode: buffers of jslot_ doubles, one per stream
     size_t jslot_ = 0;
