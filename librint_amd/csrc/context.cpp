@@ -114,6 +114,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     check(hipEventCreate(&ev0_), "hipEventCreate");
     check(hipEventCreate(&ev1_), "hipEventCreate");
     if (const char *e = getenv("LIBRINT_STREAMS")) nstreams_ = std::max(1, std::min(8, atoi(e)));
+    if (const char *e = getenv("LIBRINT_TEAM_KETS")) team_kets_ = std::max(1, std::min(64, atoi(e)));
     build_pairs();
     upload();
     compute_schwarz();
@@ -459,8 +460,13 @@ void Context::build_tasks()
                     A.stat.prim_quartets += (long long)nn * bx.npp * by.npp;
                     A.stat.flops += fl;
                     A.stat.lanes += 64LL * ((nn + 63) / 64);
-                    for (int k0 = k_begin; k0 < k_end; k0 += 64) {
-                        const int nk = std::min(64, k_end - k0);
+                    // lane-per-quartet classes take 64-ket windows (a lane per
+                    // ket, Coulomb slabs per window); the classes whose
+                    // wavefront works through its kets one at a time take
+                    // short windows, so that no task runs far longer than the rest
+                    const int win = class_has_fp32(la, lb, lc, ld) ? 64 : team_kets_;
+                    for (int k0 = k_begin; k0 < k_end; k0 += win) {
+                        const int nk = std::min(win, k_end - k0);
                         A.tasks.push_back(JKTask{p, by.first + k0, nk, 0});
                         A.cost.push_back((double)nk * bx.npp * by.npp);
                     }

@@ -189,6 +189,7 @@ private:
     // joined before the Coulomb reduction: small launches and launch tails
     // overlap
     int nstreams_ = 0;      // 0: 2 for FP64 lists, 4 for mixed ones
+    int team_kets_ = 8;     // ket window of the tasks of the team and f/g classes (LIBRINT_TEAM_KETS)
     int dominant_ = 0;      // launch with the most flops: runs first, alone
     int slab_streams_ = 1;  // per-launch slab mode: buffers of jslot_ doubles, one per stream
     size_t jslot_ = 0;
