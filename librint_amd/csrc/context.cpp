@@ -969,7 +969,8 @@ void Context::build_grad_tasks()
 {
     // Ket-major list of the quartets of build_tasks (bins X >= Y, Q_i Q_j >
     // tau, j <= i inside one bin): for each ket pair j of bin Y the bras of
-    // bin X form a prefix in Q order, cut into windows of 64.
+    // bin X form a prefix in Q order, cut into windows of 64 (team_kets_ for
+    // the classes whose Fock tasks take short windows).
     glaunch1_.clear();
     dev_free(d_gtasks1_);
     const int nb = (int)bins_.size();
@@ -988,8 +989,12 @@ void Context::build_grad_tasks()
                     if (pairs_[bx.first + mid].Q * Qk > tau_) lo = mid + 1;
                     else hi = mid;
                 }
-                for (int i0 = X == Y ? j : 0; i0 < lo; i0 += 64)
-                    v.push_back(JKTask{k, bx.first + i0, std::min(64, lo - i0), 0});
+                int la, lb, lc, ld;
+                cls_ls(bx.cls, la, lb);
+                cls_ls(by.cls, lc, ld);
+                const int win = class_has_fp32(la, lb, lc, ld) ? 64 : team_kets_;  // as build_tasks
+                for (int i0 = X == Y ? j : 0; i0 < lo; i0 += win)
+                    v.push_back(JKTask{k, bx.first + i0, std::min(win, lo - i0), 0});
             }
         }
     }

@@ -158,8 +158,8 @@ __global__ __launch_bounds__(128) void int1e_kernel(const int2 *__restrict__ pai
 // d/dC V^C = -(d/dA + d/dB) V^C.  Per primitive x_A^a exp(-alpha r_A^2):
 //   d/dA_x = 2 alpha G(a + 1_x) - a_x G(a - 1_x),  d/dalpha = -sum_x G(a + 2_x),
 //   d/dc = G / c.
-// One thread per ordered shell pair; the 1D tables hold bra powers up to
-// LI + 2.
+// One thread per ordered shell pair (per pair and point charge for V); the
+// 1D tables hold bra powers up to LI + 2.
 
 // value of the primitive integral with bra powers ia, ket powers jb
 template <int LI, int LJ>
@@ -250,8 +250,11 @@ __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict_
                                                          double *__restrict__ dS, long long s1, long long L)
 {
     using TT = DerivTables<LI, LJ>;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= npair) return;
+    // typ 2: one thread per (pair, point charge); otherwise one per pair
+    const int nsub = typ == 2 ? nnuc : 1;
+    const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tt >= (long long)npair * nsub) return;
+    const int t = (int)(tt / nsub), csel = (int)(tt - (long long)t * nsub);
     const int2 pr = pairs[t];
     const Shell1Dev A = sh[pr.x], B = sh[pr.y];  // A.pad = atom of A
     double AB[3], AB2 = 0.0;
@@ -307,7 +310,7 @@ __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict_
                 }
                 deriv_accumulate<LI, LJ>(T, typ, pref0, a, ca, A, B, ip, W, nao, dex, dco, gAt, dS, s1, L);
             } else {
-                for (int c = 0; c < nnuc; c++) {
+                for (int c = csel; c < csel + 1; c++) {
                     const double4 C = nuc[c];
                     const double PC[3] = {P[0] - C.x, P[1] - C.y, P[2] - C.z};
                     const double PC2 = PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2];
@@ -365,7 +368,10 @@ hipError_t launch_int1e_deriv(int li, int lj, const int2 *pairs, int npair, cons
 {
     if (npair <= 0) return hipSuccess;
     if (typ < 0 || typ > 2 || (dS && typ != 0)) return hipErrorInvalidValue;
-    const dim3 grid((npair + 63) / 64);
+    if (typ == 2 && nnuc <= 0) return hipSuccess;
+    const long long nthr = (long long)npair * (typ == 2 ? nnuc : 1);
+    if ((nthr + 63) / 64 > 0x7fffffffLL) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((nthr + 63) / 64));
 #define X(a, b)                                                                                                        \
     if (li == a && lj == b) {                                                                                          \
         hipLaunchKernelGGL((int1e_deriv_kernel<a, b>), grid, dim3(64), 0, st, pairs, npair, sh, env, nuc, nnuc, nao,   \
