@@ -582,18 +582,22 @@ __global__ __launch_bounds__(64) void jk_team_kernel(const JKTask *__restrict__ 
     using C = Class<LA, LB, LC, LD>;
     using S = TeamShape<LA, LB, LC, LD>;
     __shared__ double g[S::NCH * C::GS];
-    __shared__ double mab[S::NAB];
-    __shared__ double mk[S::NKET];
-    __shared__ double accs[S::NPL * 64];  // contracted outputs, read back one at a time for J/K
+    __shared__ double accs[S::NPL * 64];  // contracted block V[f], f = fa + NFA (fb + NFB (fc + NFC fd))
+    // density blocks: D_ab of the bra (per task), then D_cd, D_ac, D_bc,
+    // D_ad, D_bd of the ket (the order of the ket-side outputs below)
+    __shared__ double dl[S::NAB + S::NKET];
     if (flags && !((accept >> flags[blockIdx.x]) & 1)) return;
     const JKTask t = tasks[blockIdx.x];
     const int lane = threadIdx.x;
     const PairView bra = load_pair(B, t.bra);
     int pk[S::NPL];
     team_indices<LA, LB, LC, LD>(lane, pk);
-    for (int e = lane; e < S::NAB; e += 64) mab[e] = 0.0;
     const size_t n = (size_t)nao;
-    double *mcd = mk, *mac = mk + S::NCD, *mbc = mac + S::NAC, *mad = mbc + S::NBC, *mbd = mad + S::NAD;
+    constexpr int SC = C::NFA * C::NFB, SD = SC * C::NFC;  // strides of fc, fd in V
+    constexpr int OCD = S::NAB, OAC = OCD + S::NCD, OBC = OAC + S::NAC, OAD = OBC + S::NBC, OBD = OAD + S::NAD;
+    constexpr int NKPL = (S::NKET + 63) / 64;
+    for (int e = lane; e < S::NAB; e += 64) dl[e] = D[(bra.aoa + e / C::NFB) * n + bra.aob + e % C::NFB];
+    double jab = 0.0;  // lane < NAB: Coulomb element (fa fb) of the bra, summed over the kets
     for (int kk = 0; kk < t.nket; kk++) {
         const int ketp = t.ket0 + kk;
         const PairView ket = load_pair(B, ketp);
@@ -603,28 +607,16 @@ __global__ __launch_bounds__(64) void jk_team_kernel(const JKTask *__restrict__ 
         if (bra.sa == bra.sb) s *= 0.5;
         if (ket.sa == ket.sb) s *= 0.5;
         if (t.bra == ketp) s *= 0.5;
-        for (int e = lane; e < S::NKET; e += 64) mk[e] = 0.0;
 #pragma unroll
         for (int k = 0; k < S::NPL; k++) accs[k * 64 + lane] = acc[k];
-        __syncthreads();
-#pragma unroll 1
-        for (int k = 0; k < S::NPL; k++) {
-            const int f = lane + 64 * k;
-            if (f >= C::NF) continue;
-            const int fa = f % C::NFA, t1 = f / C::NFA;
-            const int fb = t1 % C::NFB, t2 = t1 / C::NFB;
-            const int fc = t2 % C::NFC, fd = t2 / C::NFC;
-            const size_t a = bra.aoa + fa, b = bra.aob + fb, c = ket.aoa + fc, d = ket.aob + fd;
-            const double v = s * accs[k * 64 + lane];
-            atomicAdd(&mab[fa * C::NFB + fb], 2.0 * v * D[c * n + d]);
-            atomicAdd(&mcd[fc * C::NFD + fd], 2.0 * v * D[a * n + b]);
-            atomicAdd(&mac[fa * C::NFC + fc], -0.5 * v * D[b * n + d]);
-            atomicAdd(&mbc[fb * C::NFC + fc], -0.5 * v * D[a * n + d]);
-            atomicAdd(&mad[fa * C::NFD + fd], -0.5 * v * D[b * n + c]);
-            atomicAdd(&mbd[fb * C::NFD + fd], -0.5 * v * D[a * n + c]);
-        }
-        __syncthreads();
-        for (int e = lane; e < S::NKET; e += 64) {
+        // ket-side element e: its M target (row, col) is also where its
+        // density block entry comes from (M[c][d] <- D[c][d], M[a][c] <- D[a][c], ...)
+        size_t rc[NKPL];
+#pragma unroll
+        for (int j = 0; j < NKPL; j++) {
+            const int e = lane + 64 * j;
+            rc[j] = 0;
+            if (e >= S::NKET) continue;
             size_t row, col;
             if (e < S::NCD) {
                 row = ket.aoa + e / C::NFD;
@@ -646,11 +638,54 @@ __global__ __launch_bounds__(64) void jk_team_kernel(const JKTask *__restrict__ 
                 row = bra.aob + x / C::NFD;
                 col = ket.aob + x % C::NFD;
             }
-            atomicAdd(&M[row * n + col], mk[e]);
+            rc[j] = row * n + col;
+            dl[S::NAB + e] = D[rc[j]];
+        }
+        __syncthreads();
+        // every J/K element is a small contraction of V with one density block
+        if (lane < S::NAB) {
+            const int fa = lane / C::NFB, fb = lane - fa * C::NFB;
+            double v = 0.0;
+#pragma unroll
+            for (int fd = 0; fd < C::NFD; fd++)
+#pragma unroll
+                for (int fc = 0; fc < C::NFC; fc++)
+                    v += accs[fa + C::NFA * fb + SC * fc + SD * fd] * dl[OCD + fc * C::NFD + fd];
+            jab += 2.0 * s * v;
+        }
+#pragma unroll
+        for (int j = 0; j < NKPL; j++) {
+            const int e = lane + 64 * j;
+            if (e >= S::NKET) continue;
+            // summed V indices i1 < n1 (stride vs1), i2 < n2 (stride vs2)
+            // against dl[db + i1*ds1 + i2]
+            int vb, vs1, vs2, n1, n2, db, ds1;
+            double coef = -0.5;
+            if (e < S::NCD) {  // M[c][d] += 2 s sum_ab V D_ab
+                const int fc = e / C::NFD, fd = e - fc * C::NFD;
+                vb = SC * fc + SD * fd, vs1 = 1, vs2 = C::NFA, n1 = C::NFA, n2 = C::NFB, db = 0, ds1 = C::NFB;
+                coef = 2.0;
+            } else if (e < S::NCD + S::NAC) {  // M[a][c] -= s/2 sum_bd V D_bd
+                const int x = e - S::NCD, fa = x / C::NFC, fc = x - fa * C::NFC;
+                vb = fa + SC * fc, vs1 = C::NFA, vs2 = SD, n1 = C::NFB, n2 = C::NFD, db = OBD, ds1 = C::NFD;
+            } else if (e < S::NCD + S::NAC + S::NBC) {  // M[b][c] -= s/2 sum_ad V D_ad
+                const int x = e - S::NCD - S::NAC, fb = x / C::NFC, fc = x - fb * C::NFC;
+                vb = C::NFA * fb + SC * fc, vs1 = 1, vs2 = SD, n1 = C::NFA, n2 = C::NFD, db = OAD, ds1 = C::NFD;
+            } else if (e < S::NCD + S::NAC + S::NBC + S::NAD) {  // M[a][d] -= s/2 sum_bc V D_bc
+                const int x = e - S::NCD - S::NAC - S::NBC, fa = x / C::NFD, fd = x - fa * C::NFD;
+                vb = fa + SD * fd, vs1 = C::NFA, vs2 = SC, n1 = C::NFB, n2 = C::NFC, db = OBC, ds1 = C::NFC;
+            } else {  // M[b][d] -= s/2 sum_ac V D_ac
+                const int x = e - S::NCD - S::NAC - S::NBC - S::NAD, fb = x / C::NFD, fd = x - fb * C::NFD;
+                vb = C::NFA * fb + SD * fd, vs1 = 1, vs2 = SC, n1 = C::NFA, n2 = C::NFC, db = OAC, ds1 = C::NFC;
+            }
+            double v = 0.0;
+            for (int i1 = 0; i1 < n1; i1++)
+                for (int i2 = 0; i2 < n2; i2++) v += accs[vb + vs1 * i1 + vs2 * i2] * dl[db + ds1 * i1 + i2];
+            atomicAdd(&M[rc[j]], coef * s * v);
         }
         __syncthreads();
     }
-    for (int e = lane; e < S::NAB; e += 64) atomicAdd(&M[(bra.aoa + e / C::NFB) * n + bra.aob + e % C::NFB], mab[e]);
+    if (lane < S::NAB) atomicAdd(&M[(bra.aoa + lane / C::NFB) * n + bra.aob + lane % C::NFB], jab);
 }
 
 // list variant for the team classes: one wavefront per list task
