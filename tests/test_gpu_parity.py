@@ -271,6 +271,30 @@ def test_mixed_precision_fock(gpu):
     assert abs(0.5 * np.sum(P * (Gs - G64))) < 1e-7
 
 
+@pytest.mark.parametrize("builder", ["c16h34", "w2"])
+def test_stream_counts_agree(gpu, builder, monkeypatch):
+    """The class launches after the first are dealt over several streams;
+    any stream count gives the same G up to the summation order of the
+    atomics (FP64 and mixed lists, slab and team classes)."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("stream counts", builder)
+    m = {"c16h34": mol.c16h34_631g, "w2": lambda: mol.water_cluster(2)}[builder]()
+    P = _core_density(m)
+    out = {}
+    for ns in ("1", "4"):
+        monkeypatch.setenv("LIBRINT_STREAMS", ns)
+        ctx = FockContext(m, schwarz_tol=1e-12)
+        out[ns] = ctx.fock(P)
+        ctx.set_precision(1, 1e-5)
+        out[ns + "m"] = ctx.fock(P)
+        ctx.close()
+    scale = np.abs(out["1"]).max()
+    assert np.abs(out["1"] - out["4"]).max() < 1e-13 * scale
+    assert np.abs(out["1m"] - out["4m"]).max() < 1e-13 * scale
+    assert np.abs(out["1m"] - out["1"]).max() < 1e-6
+
+
 def test_mixed_precision_d_shells(gpu):
     from librint_amd import mol
     from librint_amd.context import FockContext
