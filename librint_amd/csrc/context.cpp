@@ -391,6 +391,9 @@ static double flops_prim(int la, int lb, int lc, int ld)
 
 void Context::build_tasks()
 {
+    // the lists below replace device buffers that launches of an earlier
+    // build (on this context's extra streams too) may still be reading
+    if (d_tasks_) check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     const int nb = (int)bins_.size();
     launches_.clear();
     stats_ = Stats();
@@ -972,6 +975,7 @@ void Context::build_grad_tasks()
     // bin X form a prefix in Q order, cut into windows of 64 (team_kets_ for
     // the classes whose Fock tasks take short windows).
     glaunch1_.clear();
+    if (d_gtasks1_) check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     dev_free(d_gtasks1_);
     const int nb = (int)bins_.size();
     std::map<std::pair<int, int>, std::vector<JKTask>> acc;  // (bra class, ket class)
