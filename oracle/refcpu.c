@@ -15,6 +15,8 @@
  *   rc_int1e           S, T, V                 reference src/scf.rs:160-213, src/g1e.rs, src/cint1e.rs
  *   rc_energyfast      E[P]                    reference src/scf.rs:827-956
  *   rc_density/rc_scf  Roothaan SCF from P=I   reference src/scf.rs:572-799, 958-969
+ *   rc_schwarz, rc_fock2e_screened  the same G from canonical Schwarz-screened
+ *                      quartets, 8-fold symmetry, OpenMP (a faster CPU comparator)
  *
  * Numerical choices of this restatement (it does not reuse any table of the
  * reference):
@@ -952,4 +954,117 @@ void rc_fock2e_sample(double *G, const double *P, const int *quartets, long nq, 
     }
     free(buf);
     free(off);
+}
+
+/* ------------------------------------------- screened symmetric G -- */
+/* A second CPU comparator for the Fock build, with the shortcuts a tuned CPU
+ * code takes that the reference loop (rc_fock2e, every ordered quartet) does
+ * not: 8-fold permutational symmetry, Schwarz screening and threads.  Its
+ * quartet set is the one the HIP path evaluates, so on the same inputs the two
+ * agree to rounding; against rc_fock2e it differs by the screened quartets. */
+
+/* Q[ij] = sqrt(max |(ij|ij)|) over the block, ij = i(i+1)/2 + j, i >= j. */
+void rc_schwarz(double *Q, const int *atm, int natm, const int *bas, int nbas, const double *env, int nthreads)
+{
+    double u[NRMAX], w[NRMAX];
+    rc_rys_roots(1, 1.0, u, w); /* fills the static quadrature before the threads start */
+    const long npair = (long)nbas * (nbas + 1) / 2;
+#pragma omp parallel num_threads(nthreads)
+    {
+        double *buf = (double *)malloc(sizeof(double) * 50625);
+#pragma omp for schedule(dynamic, 16)
+        for (long ij = 0; ij < npair; ij++) {
+            int i = (int)((sqrt(8.0 * (double)ij + 1.0) - 1.0) / 2.0);
+            while ((long)i * (i + 1) / 2 > ij) i--;
+            while ((long)(i + 1) * (i + 2) / 2 <= ij) i++;
+            const int j = (int)(ij - (long)i * (i + 1) / 2);
+            int shls[4] = {i, j, i, j}, d[4];
+            rc_int2e_shell(buf, shls, atm, natm, bas, nbas, env);
+            quartet_dims(bas, shls, d);
+            const long n = (long)d[0] * d[1] * d[2] * d[3];
+            double m = 0.0;
+            for (long t = 0; t < n; t++) m = fmax(m, fabs(buf[t]));
+            Q[ij] = sqrt(m);
+        }
+        free(buf);
+    }
+}
+
+/* G = J - K/2 (the convention of rc_fock2e, symmetric P) from the canonical
+ * quartets i >= j, k >= l, ij >= kl with Q[ij] Q[kl] > tau, each evaluated
+ * once and added to its eight images with weight
+ * 2^-([i=j] + [k=l] + [ij=kl]).  Only bra pairs ij with ij % stride == phase
+ * are visited (stride 1: the whole build; larger strides time a bounded
+ * sample).  Threads keep private G's, summed at the end.  Returns the number
+ * of quartets evaluated. */
+long rc_fock2e_screened(double *G, const double *P, const double *Q, double tau, int stride, int phase,
+                        const int *atm, int natm, const int *bas, int nbas, const double *env, int nthreads)
+{
+    double u[NRMAX], w[NRMAX];
+    rc_rys_roots(1, 1.0, u, w);
+    int *off = (int *)malloc(sizeof(int) * (nbas + 1));
+    ao_offsets(bas, nbas, off);
+    const size_t n = off[nbas];
+    memset(G, 0, sizeof(double) * n * n);
+    const long npair = (long)nbas * (nbas + 1) / 2;
+    int *pi = (int *)malloc(sizeof(int) * npair), *pj = (int *)malloc(sizeof(int) * npair);
+    for (int i = 0, c = 0; i < nbas; i++)
+        for (int j = 0; j <= i; j++, c++) {
+            pi[c] = i;
+            pj[c] = j;
+        }
+    long count = 0;
+#pragma omp parallel num_threads(nthreads) reduction(+ : count)
+    {
+        double *buf = (double *)malloc(sizeof(double) * 50625);
+        double *Gt = (double *)calloc(n * n, sizeof(double));
+#pragma omp for schedule(dynamic, 1)
+        for (long ij = phase; ij < npair; ij += stride) {
+            if (Q[ij] == 0.0) continue;
+            for (long kl = 0; kl <= ij; kl++) {
+                if (!(Q[ij] * Q[kl] > tau)) continue;
+                int shls[4] = {pi[ij], pj[ij], pi[kl], pj[kl]}, d[4];
+                rc_int2e_shell(buf, shls, atm, natm, bas, nbas, env);
+                quartet_dims(bas, shls, d);
+                double s = 1.0;
+                if (shls[0] == shls[1]) s *= 0.5;
+                if (shls[2] == shls[3]) s *= 0.5;
+                if (ij == kl) s *= 0.5;
+                count++;
+                int c = 0;
+                for (int l = 0; l < d[3]; l++)
+                    for (int k = 0; k < d[2]; k++)
+                        for (int j = 0; j < d[1]; j++)
+                            for (int i = 0; i < d[0]; i++) {
+                                const size_t a = off[shls[0]] + i, b = off[shls[1]] + j;
+                                const size_t cc = off[shls[2]] + k, dd = off[shls[3]] + l;
+                                const double v = s * buf[c++];
+                                /* Coulomb: the four images with bra (a,b) or (b,a) add P_cd, ... */
+                                const double jab = 2.0 * v * P[cc * n + dd], jcd = 2.0 * v * P[a * n + b];
+                                Gt[a * n + b] += jab;
+                                Gt[b * n + a] += jab;
+                                Gt[cc * n + dd] += jcd;
+                                Gt[dd * n + cc] += jcd;
+                                /* exchange: G[p][s] -= P[q][r] v / 2 for every image (p q|r s) */
+                                const double h = 0.5 * v;
+                                Gt[a * n + dd] -= h * P[b * n + cc];
+                                Gt[b * n + dd] -= h * P[a * n + cc];
+                                Gt[a * n + cc] -= h * P[b * n + dd];
+                                Gt[b * n + cc] -= h * P[a * n + dd];
+                                Gt[cc * n + b] -= h * P[dd * n + a];
+                                Gt[dd * n + b] -= h * P[cc * n + a];
+                                Gt[cc * n + a] -= h * P[dd * n + b];
+                                Gt[dd * n + a] -= h * P[cc * n + b];
+                            }
+            }
+        }
+#pragma omp critical
+        for (size_t t = 0; t < n * n; t++) G[t] += Gt[t];
+        free(Gt);
+        free(buf);
+    }
+    free(pi);
+    free(pj);
+    free(off);
+    return count;
 }

@@ -47,6 +47,10 @@ def lib():
                                 ctypes.c_double, ctypes.c_int)
         _lib.rc_scf.restype = ctypes.c_double
         _lib.rc_nao.argtypes = (ip, ctypes.c_int, ctypes.c_int)
+        _lib.rc_schwarz.argtypes = (dp, ip, ctypes.c_int, ip, ctypes.c_int, dp, ctypes.c_int)
+        _lib.rc_fock2e_screened.argtypes = (dp, dp, dp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ip,
+                                            ctypes.c_int, ip, ctypes.c_int, dp, ctypes.c_int)
+        _lib.rc_fock2e_screened.restype = ctypes.c_long
         _lib.rc_nao.restype = ctypes.c_int
     return _lib
 
@@ -122,6 +126,40 @@ def fock2e_sample(m, P, quartets):
     G = np.zeros((n, n))
     lib().rc_fock2e_sample(_d(G), _d(P), _i(q), len(q), _i(atm), natm, _i(bas), nbas, _d(env))
     return G
+
+
+def host_threads():
+    """Threads for the threaded comparator: the cores this process may use,
+    at most OMP_NUM_THREADS when that is set."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def schwarz(m, nthreads=None):
+    """Q[ij] = sqrt(max|(ij|ij)|) per shell pair, ij = i(i+1)/2 + j."""
+    atm, bas, env, natm, nbas = _mol(m)
+    Q = np.zeros(nbas * (nbas + 1) // 2)
+    lib().rc_schwarz(_d(Q), _i(atm), natm, _i(bas), nbas, _d(env), nthreads or host_threads())
+    return Q
+
+
+def fock2e_screened(m, P, tau, Q=None, stride=1, phase=0, nthreads=None):
+    """G = J - K/2 from the canonical quartets with Q_ij Q_kl > tau (8-fold
+    symmetry, threads); returns (G, quartets evaluated).  stride/phase visit
+    the bra pairs ij with ij % stride == phase only."""
+    atm, bas, env, natm, nbas = _mol(m)
+    n = m.nao()
+    if Q is None:
+        Q = schwarz(m, nthreads)
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    G = np.zeros((n, n))
+    cnt = lib().rc_fock2e_screened(_d(G), _d(P), _d(Q), float(tau), int(stride), int(phase), _i(atm), natm,
+                                   _i(bas), nbas, _d(env), nthreads or host_threads())
+    return G, int(cnt)
 
 
 def int1e(m, typ):

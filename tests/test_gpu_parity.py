@@ -112,6 +112,21 @@ def test_fock_c16h34_block_density(gpu):
     assert np.abs(G - G.T).max() < 1e-13 * np.abs(G).max()
 
 
+def test_fock_vs_screened_oracle_same_quartets(gpu):
+    """At Schwarz 1e-12 the GPU build and the oracle's screened symmetric
+    build visit the same canonical quartets, so G agrees to rounding."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("fock C4H10 vs screened oracle, tau 1e-12")
+    m = mol.alkane_631g(4)
+    ctx = FockContext(m, schwarz_tol=1e-12)
+    P = _rand_sym(m.nao(), 11)
+    G = ctx.fock(P)
+    Gr, cnt = refcpu.fock2e_screened(m, P, 1e-12)
+    assert cnt == ctx.stats()["quartets"]
+    assert np.abs(G - Gr).max() < 1e-13 * np.abs(Gr).max()
+
+
 def test_fock_linearity_and_screening(gpu):
     from librint_amd import mol
     from librint_amd.context import FockContext

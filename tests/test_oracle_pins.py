@@ -134,6 +134,29 @@ def test_fock_direct_matches_tensor():
     assert np.abs(G - (J - 0.5 * K)).max() < 1e-13
 
 
+def test_screened_symmetric_fock_matches_reference_loop():
+    # the threaded 8-fold-symmetry comparator against the reference-loop
+    # fixture (every ordered quartet, src/scf.rs:316-410): same G at tau = 0,
+    # partitions of the bra pairs add up, thread count does not matter
+    from librint_amd import mol
+    d = np.load(os.path.join(GOLD, "fock_c2h6_631g.npz"))
+    m = mol.alkane_631g(2)
+    Q = refcpu.schwarz(m, 4)
+    G, cnt = refcpu.fock2e_screened(m, d["P"], 0.0, Q=Q, nthreads=4)
+    assert np.abs(G - d["G"]).max() < 1e-13 * np.abs(d["G"]).max()
+    nz = np.count_nonzero(Q)
+    assert cnt == nz * (nz + 1) // 2
+    G1, _ = refcpu.fock2e_screened(m, d["P"], 0.0, Q=Q, nthreads=1)
+    assert np.abs(G1 - G).max() < 1e-14 * np.abs(G).max()
+    parts = [refcpu.fock2e_screened(m, d["P"], 0.0, Q=Q, stride=3, phase=k, nthreads=2) for k in range(3)]
+    assert sum(c for _, c in parts) == cnt
+    assert np.abs(sum(g for g, _ in parts) - G).max() < 1e-14 * np.abs(G).max()
+    # a screened build: fewer quartets, error within tau * sum|P| per element
+    Gs, cs = refcpu.fock2e_screened(m, d["P"], 1e-8, Q=Q, nthreads=4)
+    assert cs < cnt
+    assert np.abs(Gs - d["G"]).max() < 1e-8 * np.abs(d["P"]).sum()
+
+
 def test_planted_mistake_is_caught():
     # a wrong sign in the exchange term must fail the tensor comparison above
     m = ref_mol("h2o", "sto3g", 10)
