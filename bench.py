@@ -278,6 +278,8 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "fock_s_per_iter": ms_per_step * 1e-3,
+        "integrals_per_s": st["integrals"] / (ms_per_step * 1e-3),
+        "prim_quartets_per_s": st["prim_quartets"] / (ms_per_step * 1e-3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
