@@ -98,6 +98,57 @@ def cpu_threaded(m, P, tau, seconds, stride=16):
     return done / used, nth, done, used, phases
 
 
+def bench_shares(args, m, P, ctx):
+    """One device's share of the Fock build, for every share of a
+    G-device split (the deal of set_shard): each share timed on its own at
+    its real shape, the whole build beside them, and the sum of the shares'
+    G's checked against the whole G."""
+    import torch
+    dev = torch.device("cuda:0")
+    Pd = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
+    Gd = torch.empty_like(Pd)
+
+    def timed():
+        for _ in range(args.warmup):
+            ctx.fock_device(Pd, Gd)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.fock_device(Pd, Gd)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps * 1e3
+
+    whole_ms = timed()
+    G = Gd.cpu().numpy()
+    whole = ctx.stats()
+    shares, Gsum = [], np.zeros_like(G)
+    for k in range(args.shares):
+        ctx.set_shard(args.shares, k)
+        ms = timed()
+        Gsum += Gd.cpu().numpy()
+        st = ctx.stats()
+        shares.append({"share": k, "ms": ms, "quartets": int(st["quartets"]),
+                       "prim_quartets": int(st["prim_quartets"]), "flops": st["flops"]})
+    ctx.set_shard(1, 0)
+    ms = np.array([x["ms"] for x in shares])
+    fl = np.array([x["flops"] for x in shares])
+    line = {
+        "metric": "RHF Fock-build s/iter, one device's share of a %d-device split" % args.shares,
+        "value": float(ms.max()) * 1e-3, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": float(ms.max()), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic geometry, core-Hamiltonian density",
+        "config": {"workload": args.workload, "nao": m.nao(), "schwarz": ctx.stats()["schwarz_tol"],
+                   "shares": args.shares, "quartets": int(whole["quartets"])},
+        "whole_ms": whole_ms,
+        "share_ms_max_over_mean": float(ms.max() / ms.mean()),
+        "share_flops_max_over_mean": float(fl.max() / fl.mean()),
+        "speedup_if_shares_ran_in_parallel": whole_ms / float(ms.max()),
+        "max_abs_sum_of_shares_minus_whole": float(np.abs(Gsum - G).max()),
+        "shares": shares,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -186,6 +237,8 @@ def main():
     ap.add_argument("--tau-mix-compare", type=float, nargs="*", default=[1e-5, 1e-3],
                     help="FP64 runs also time the mixed-precision build at these thresholds")
     ap.add_argument("--no-mixed-compare", action="store_true")
+    ap.add_argument("--shares", type=int, default=1,
+                    help="time each device's share of a G-device split of the Fock build on this one GPU")
     ap.add_argument("--grad", action="store_true",
                     help="time the analytic RHF gradient (adjoint ERI kernel) instead of the Fock build")
     ap.add_argument("--copy-mb", type=float, default=0.0,
@@ -217,6 +270,8 @@ def main():
     m, tau = workload(args.workload)
     P = core_guess_density(m)
     ctx = FockContext(m, schwarz_tol=tau)
+    if args.shares > 1:
+        return bench_shares(args, m, P, ctx)
     mixed = args.precision == "mixed"
     G64 = None
     if mixed or args.tau_skip > 0:

@@ -97,6 +97,17 @@ void librint_ctx_destroy(librint_ctx *ctx);
 int librint_ctx_nao(const librint_ctx *ctx);
 /* Set the Schwarz threshold (default 1e-14); returns 0 on success. */
 int librint_ctx_set_schwarz(librint_ctx *ctx, double tol);
+/* Restrict the context's Fock builds to one device's share of the work when
+ * nshards devices split it (no counterpart in the reference, which runs on one
+ * host; for a multi-device caller with one context per device).  The groups
+ * of quartet tasks of every class launch are dealt heaviest first, each to the
+ * share with the least primitive-quartet cost so far (ties to the lowest
+ * index: every device computes the same deal), and this context keeps share
+ * `shard`.  librint_fock* then return that share's G; the G's of the nshards
+ * shares sum to the whole build (one all-reduce).  nshards = 1 (default): the
+ * whole build.  The SCF and gradient calls refuse a context with nshards > 1.
+ * Returns 0 on success, -1 for shard outside [0, nshards). */
+int librint_ctx_set_shard(librint_ctx *ctx, int nshards, int shard);
 
 /* Precision of the Fock build.  mode 0: FP64 everywhere (default).
  * mode 1: mixed -- quartets with Q_ab*Q_cd*dmax < tau_mix run their Rys

@@ -50,6 +50,8 @@ library.librint_ctx_nao.argtypes = (ctypes.c_void_p,)
 library.librint_ctx_nao.restype = ctypes.c_int
 library.librint_ctx_set_schwarz.argtypes = (ctypes.c_void_p, ctypes.c_double)
 library.librint_ctx_set_schwarz.restype = ctypes.c_int
+library.librint_ctx_set_shard.argtypes = (ctypes.c_void_p, ctypes.c_int, ctypes.c_int)
+library.librint_ctx_set_shard.restype = ctypes.c_int
 library.librint_fock.argtypes = (ctypes.c_void_p, _dp, _dp)
 library.librint_fock.restype = ctypes.c_int
 library.librint_fock_device.argtypes = (ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
@@ -101,5 +103,5 @@ EXPORTED = (
     "librint_ctx_class_stats", "librint_ctx_set_precision", "librint_ctx_flag_counts",
     "librint_scf", "librint_last_error", "librint_ctx_gradient", "grad_nuc_c", "librint_scf_ex",
     "librint_mlp_gemm", "librint_mlp_relu_bwd", "librint_mlp_colsum", "librint_mlp_mse", "librint_mlp_adam",
-    "librint_mlp_to_bf16",
+    "librint_mlp_to_bf16", "librint_ctx_set_shard",
 )

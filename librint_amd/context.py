@@ -30,6 +30,12 @@ class FockContext:
     def __del__(self):
         self.close()
 
+    def set_shard(self, nshards, shard):
+        """Keep one device's share of the Fock build (nshards devices; the
+        shares' G's sum to the whole build)."""
+        if library.librint_ctx_set_shard(self.h, int(nshards), int(shard)) != 0:
+            raise RuntimeError(utils.last_error())
+
     def set_schwarz(self, tol):
         if library.librint_ctx_set_schwarz(self.h, float(tol)) != 0:
             raise RuntimeError(utils.last_error())

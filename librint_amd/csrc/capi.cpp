@@ -443,6 +443,21 @@ void librint_ctx_destroy(librint_ctx *ctx) { delete ctx; }
 
 int librint_ctx_nao(const librint_ctx *ctx) { return ctx ? ctx->ctx->nao() : -1; }
 
+int librint_ctx_set_shard(librint_ctx *ctx, int nshards, int shard)
+{
+    try {
+        if (!ctx) {
+            set_err("null context");
+            return -1;
+        }
+        ctx->ctx->set_shard(nshards, shard);
+        return 0;
+    } catch (std::exception &e) {
+        set_err(e.what());
+        return -1;
+    }
+}
+
 int librint_ctx_set_schwarz(librint_ctx *ctx, double tol)
 {
     try {

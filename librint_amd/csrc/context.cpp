@@ -115,6 +115,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     check(hipEventCreate(&ev1_), "hipEventCreate");
     if (const char *e = getenv("LIBRINT_STREAMS")) nstreams_ = std::max(1, std::min(8, atoi(e)));
     if (const char *e = getenv("LIBRINT_TEAM_KETS")) team_kets_ = std::max(1, std::min(64, atoi(e)));
+    if (const char *e = getenv("LIBRINT_PRIM_SPLIT")) prim_split_ = std::max(0, atoi(e));
     build_pairs();
     upload();
     compute_schwarz();
@@ -363,6 +364,15 @@ void Context::set_schwarz(double tau)
     gtasks_valid_ = false;
 }
 
+void Context::set_shard(int nshards, int shard)
+{
+    if (nshards < 1 || shard < 0 || shard >= nshards) throw std::invalid_argument("shard must satisfy 0 <= shard < nshards");
+    nshards_ = nshards;
+    shard_ = shard;
+    build_tasks();
+    gtasks_valid_ = false;
+}
+
 // Per primitive quartet flop count of the fp64 kernel's algorithm (the
 // definition used for roofline numbers): root evaluation, recurrence
 // coefficients, VRR, HRR and the gout products, FMA counted as 2 flops.
@@ -468,10 +478,16 @@ void Context::build_tasks()
                     // wavefront works through its kets one at a time take
                     // short windows, so that no task runs far longer than the rest
                     const int win = class_has_fp32(la, lb, lc, ld) ? 64 : team_kets_;
+                    int parts = 1;
+                    if (prim_split_ > 0 && class_has_fp32(la, lb, lc, ld) && bx.npp < 65536)
+                        parts = std::min(bx.npp, (bx.npp * by.npp + prim_split_ - 1) / prim_split_);
                     for (int k0 = k_begin; k0 < k_end; k0 += win) {
                         const int nk = std::min(win, k_end - k0);
-                        A.tasks.push_back(JKTask{p, by.first + k0, nk, 0});
-                        A.cost.push_back((double)nk * bx.npp * by.npp);
+                        for (int q = 0; q < parts; q++) {
+                            const int b0 = bx.npp * q / parts, b1 = bx.npp * (q + 1) / parts;
+                            A.tasks.push_back(JKTask{p, by.first + k0, nk, parts > 1 ? (b0 | (b1 << 16)) : 0});
+                            A.cost.push_back((double)nk * (b1 - b0) * by.npp);
+                        }
                     }
                 }
             }
@@ -511,9 +527,61 @@ void Context::build_tasks()
                          [](const std::pair<double, JKGroup> &x, const std::pair<double, JKGroup> &y) {
                              return x.first > y.first;
                          });
+        int la, lb, lc, ld;
+        cls_ls(std::get<0>(kv.first), la, lb);
+        cls_ls(std::get<1>(kv.first), lc, ld);
+        if (nshards_ > 1) {
+            // deal the groups (heaviest first, each to the least loaded
+            // share; ties to the lowest index, so every share computes the
+            // same deal) and keep this share's
+            std::vector<double> load(nshards_, 0.0);
+            std::vector<std::pair<double, JKGroup>> mine;
+            for (auto &g : A.groups) {
+                const int s = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                load[s] += g.first;
+                if (s == shard_) mine.push_back(g);
+            }
+            A.groups.swap(mine);
+            // statistics of the kept tasks
+            const double fp = flops_prim(la, lb, lc, ld);
+            const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
+            ClassLaunch S;
+            for (auto &g : A.groups)
+                for (int t = g.second.first; t < g.second.first + g.second.count; t++) {
+                    const JKTask &T = A.tasks[t];
+                    int b0, b1;
+                    task_prims(T, pairs_[T.bra].npp, b0, b1);
+                    const long long pq = (long long)T.nket * (b1 - b0) * pairs_[T.ket0].npp;
+                    if (b0 == 0) S.quartets += T.nket;  // a split quartet counts once
+                    S.prim_quartets += pq;
+                    S.flops += (double)pq * fp + (b0 == 0 ? 12.0 * nf * T.nket : 0.0);
+                    if (b0 == 0) S.lanes += 64;
+                }
+            stats_.quartets += S.quartets - A.stat.quartets;
+            stats_.prim_quartets += S.prim_quartets - A.stat.prim_quartets;
+            stats_.integrals += (long long)nf * (S.quartets - A.stat.quartets);
+            stats_.flops += S.flops - A.stat.flops;
+            A.stat.quartets = S.quartets;
+            A.stat.prim_quartets = S.prim_quartets;
+            A.stat.flops = S.flops;
+            A.stat.lanes = S.lanes;
+        }
+        if (getenv("LIBRINT_DEBUG")) {
+            double tot = 0.0, gmax = 0.0, tmax = 0.0;
+            for (auto &g : A.groups) {
+                tot += g.first;
+                gmax = std::max(gmax, g.first);
+                for (int t = g.second.first; t < g.second.first + g.second.count; t++) tmax = std::max(tmax, A.cost[t]);
+            }
+            fprintf(stderr, "librint: launch (%d%d|%d%d)%s tasks %zu groups %zu cost %.4g max group %.4g max task %.4g\n",
+                    la, lb, lc, ld, std::get<2>(kv.first) ? " fp32" : "", A.tasks.size(), A.groups.size(), tot, gmax,
+                    tmax);
+        }
         ClassLaunch L = A.stat;
-        cls_ls(std::get<0>(kv.first), L.la, L.lb);
-        cls_ls(std::get<1>(kv.first), L.lc, L.ld);
+        L.la = la;
+        L.lb = lb;
+        L.lc = lc;
+        L.ld = ld;
         L.fp32 = std::get<2>(kv.first);
         L.first = (int)tasks.size();
         L.gfirst = (int)groups.size();
@@ -656,8 +724,10 @@ void Context::flag_counts(long long *q3, long long *p3)
         for (int t = L.first; t < L.first + L.count; t++) {
             const JKTask &T = h_tasks_[t];
             const int f = fl[t] == 2 ? 2 : L.fp32;
-            q3[f] += T.nket;
-            p3[f] += (long long)T.nket * pairs_[T.bra].npp * pairs_[T.ket0].npp;
+            int b0, b1;
+            task_prims(T, pairs_[T.bra].npp, b0, b1);
+            if (b0 == 0) q3[f] += T.nket;  // a split quartet counts once
+            p3[f] += (long long)T.nket * (b1 - b0) * pairs_[T.ket0].npp;
         }
 }
 
@@ -1021,6 +1091,7 @@ void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
 {
     for (const Shell &S : shells_)
         if (S.l > 2) throw std::invalid_argument("analytic gradients cover l <= 2");
+    if (nshards_ > 1) throw std::invalid_argument("analytic gradients need the whole build (nshards = 1)");
     // per-shell primitive slots: exponent and coefficient derivatives are
     // accumulated per (shell, primitive) and folded onto env entries here
     const int ns = (int)shells_.size();

@@ -60,6 +60,14 @@ public:
     // Screening threshold on Q_ij * Q_kl; rebuilds the task lists.
     void set_schwarz(double tau);
     double schwarz() const { return tau_; }
+    // One device's share of the Fock build when nshards devices split it:
+    // the groups of every class launch are dealt, heaviest first, each to
+    // the share with the least primitive-quartet cost so far, and this
+    // context keeps share `shard` (0 <= shard < nshards).  fock_* then
+    // return that share's G; the G's of all shares sum to the full build.
+    // nshards = 1 (default) is the whole build.  Rebuilds the task lists.
+    void set_shard(int nshards, int shard);
+    int nshards() const { return nshards_; }
     // Precision mode 0: FP64 everywhere.  Mode 1 (mixed): quartets with
     // Q_ab Q_cd dmax < tau_mix use FP32 recurrences, the split being made
     // once when the task lists are built (dmax <= 0: max|D| of the next
@@ -109,6 +117,7 @@ private:
     std::vector<Shell> shells_;
     int nao_ = 0;
     double tau_ = 1e-14;
+    int nshards_ = 1, shard_ = 0;
 
     // pair list in bin order (bins = (class, npp), Q descending inside a bin)
     struct PairH {
@@ -190,6 +199,12 @@ private:
     // overlap
     int nstreams_ = 0;      // 0: 2 for FP64 lists, 4 for mixed ones
     int team_kets_ = 8;     // ket window of the tasks of the team and f/g classes (LIBRINT_TEAM_KETS)
+    // lane-per-quartet classes: a task whose lanes would each evaluate more
+    // than prim_split_ primitive quartets is split by bra primitive pairs
+    // (LIBRINT_PRIM_SPLIT, default 48; 0 = never), so that the heaviest
+    // quartets (up to 36 x 36 primitive pairs on one lane) do not end their
+    // launch as one wavefront's long tail
+    int prim_split_ = 48;
     int dominant_ = 0;      // launch with the most flops: runs first, alone
     int slab_streams_ = 1;  // per-launch slab mode: buffers of jslot_ doubles, one per stream
     size_t jslot_ = 0;

@@ -165,6 +165,10 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
     // whole task before one set of atomics.
     const GPair sd = gload_pair(B, t.bra);
     const int2 si0 = shell_info[sd.sa], si1 = shell_info[sd.sb];
+    // SIDE 0 runs over the Fock build's task list, whose heavy quartets may
+    // be split by bra primitive pairs; the ket-major list is never split
+    int o_begin = 0, o_end = sd.npp;
+    if (SIDE == 0) task_prims(t, sd.npp, o_begin, o_end);
     const size_t n = (size_t)nao;
     const int niter = (t.nket + S::NT - 1) / S::NT;
     // nuclear derivatives of the two centres (per-lane partial sums); the
@@ -212,7 +216,7 @@ __global__ __launch_bounds__(64) void grad_kernel(const JKTask *__restrict__ tas
             }
         }
         const double rxy[3] = {sd.rb[0] - ot.rb[0], sd.rb[1] - ot.rb[1], sd.rb[2] - ot.rb[2]};
-        for (int o = 0; o < sd.npp; o++) {
+        for (int o = o_begin; o < o_end; o++) {
             const PrimPair po = B.pp[sd.off + o];
             const double al0 = po.pad0, al1 = po.pad1;  // exponents of the two centres
             double e0 = 0.0, e1 = 0.0, val = 0.0;

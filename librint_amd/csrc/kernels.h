@@ -51,13 +51,28 @@ __device__ __forceinline__ PairView load_pair(const DevBasis &B, int p)
 }
 
 // One wavefront of the Fock kernel: bra pair x kets [ket0, ket0 + nket) of
-// the pair list (all kets of one bin).
+// the pair list (all kets of one bin).  prims = 0: every primitive pair of
+// the bra; else the bra's primitive pairs [prims & 0xffff, prims >> 16) only
+// (a heavy quartet's primitive sum split over several tasks, whose J/K
+// contributions add up).
 struct JKTask {
     int bra;
     int ket0;
     int nket;
-    int pad;
+    int prims;
 };
+
+// Bra primitive-pair range [b0, b1) of a task whose bra has npp of them.
+__host__ __device__ inline void task_prims(const JKTask &t, int npp, int &b0, int &b1)
+{
+    if (t.prims) {
+        b0 = t.prims & 0xffff;
+        b1 = t.prims >> 16;
+    } else {
+        b0 = 0;
+        b1 = npp;
+    }
+}
 
 // All tasks of one bra pair within one class-pair launch, [first, first+count).
 struct JKGroup {

@@ -75,9 +75,11 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
     const T rcd[3] = {(T)ket.rab[0], (T)ket.rab[1], (T)ket.rab[2]};
     // every ket of a task comes from one bin, so the primitive-pair count is
     // the same in all lanes: make the loop bounds scalar for the compiler
+    int b0, b1;
+    task_prims(t, bra.npp, b0, b1);
     const int knpp = __builtin_amdgcn_readfirstlane(ket.npp);
-    const int bnpp = __builtin_amdgcn_readfirstlane(bra.npp);
-    const int boff = __builtin_amdgcn_readfirstlane(bra.off);
+    const int bnpp = __builtin_amdgcn_readfirstlane(b1 - b0);
+    const int boff = __builtin_amdgcn_readfirstlane(bra.off + b0);
     // Light classes (s on both HRR targets, NF <= 9) are software pipelined:
     // the next primitive-pair records load while the current primitive
     // quartet is evaluated, hiding their latency.  Heavier classes keep the

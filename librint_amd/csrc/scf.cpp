@@ -197,6 +197,7 @@ double energy_fast(Context &ctx, const Int1eInput &in, const double *P, const do
 ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double conv, const ScfOptions &o)
 {
     ScfOut out;
+    if (ctx.nshards() > 1) throw std::invalid_argument("the SCF needs the whole Fock build (nshards = 1)");
     const int n = ctx.nao();
     const size_t nn = (size_t)n * n;
     if (in.nao != n) throw std::runtime_error("basis mismatch between 1e and 2e inputs");

@@ -127,6 +127,36 @@ def test_fock_vs_screened_oracle_same_quartets(gpu):
     assert np.abs(G - Gr).max() < 1e-13 * np.abs(Gr).max()
 
 
+def test_fock_shards_sum_to_whole_build(gpu):
+    """One device's share of the build (the deal a multi-device run would
+    make): the shares' G's sum to the whole G, their quartets to the whole
+    count, and their costs are balanced."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("fock C16H34 shares of 4")
+    m = mol.c16h34_631g()
+    ctx = FockContext(m, schwarz_tol=1e-12)
+    P = _rand_sym(m.nao(), 12)
+    G = ctx.fock(P)
+    whole = ctx.stats()
+    parts, st = [], []
+    for k in range(4):
+        ctx.set_shard(4, k)
+        parts.append(ctx.fock(P))
+        st.append(ctx.stats())
+    assert sum(s["quartets"] for s in st) == whole["quartets"]
+    assert sum(s["prim_quartets"] for s in st) == whole["prim_quartets"]
+    assert abs(sum(s["flops"] for s in st) - whole["flops"]) < 1e-9 * whole["flops"]
+    assert np.abs(sum(parts) - G).max() < 1e-13 * np.abs(G).max()
+    assert all(np.abs(g).max() > 0 for g in parts)
+    fl = np.array([s["flops"] for s in st])
+    assert fl.max() / fl.mean() < 1.05
+    with pytest.raises(RuntimeError):
+        ctx.set_shard(4, 4)
+    ctx.set_shard(1, 0)
+    assert np.abs(ctx.fock(P) - G).max() < 1e-13 * np.abs(G).max()
+
+
 def test_fock_linearity_and_screening(gpu):
     from librint_amd import mol
     from librint_amd.context import FockContext
