@@ -110,7 +110,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     nao_ = ao;
     no_group_ = getenv("LIBRINT_NO_GROUP") != nullptr;
     no_jslab_ = getenv("LIBRINT_NO_JSLAB") != nullptr;
-    if (const char *e = getenv("LIBRINT_DROW")) drow_mode_ = atoi(e) != 0 ? 1 : 0;
+    if (const char *e = getenv("LIBRINT_DROW")) drow_mode_ = std::max(0, std::min(2, atoi(e)));
     check(hipEventCreate(&ev0_), "hipEventCreate");
     check(hipEventCreate(&ev1_), "hipEventCreate");
     if (const char *e = getenv("LIBRINT_STREAMS")) nstreams_ = std::max(1, std::min(8, atoi(e)));
@@ -814,11 +814,19 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         int *ctr = d_counters_ + 2 * k;
         const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, false);
         const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
-        // staging the bra's density rows in LDS (LIBRINT_DROW=1) halves the
-        // row space per workgroup; measured slower overall on a 992-function
-        // cluster, so it is off by default
+        // Staging the bra's density rows in LDS (DROW) doubles the row space
+        // per workgroup but turns the exchange update's gathers of D into LDS
+        // reads.  Default (mode 2): stage them where a quartet has fewer than
+        // 4 primitive quartets on average (the exchange update is then a large
+        // part of the work) and two workgroups still fit a CU; LIBRINT_DROW=0
+        // never, 1 wherever it fits.  Measured: (H2O)32 48.5 -> 46.8 ms, the
+        // contracted C16H34 lists unchanged (none qualifies; staging them all
+        // was slower, 1.62 -> 1.65 ms).
+        const size_t lds_drow = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, true);
         bool drow = drow_mode_ == 1;
-        drow = drow && grouped && jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, true) <= (size_t)max_lds_;
+        if (drow_mode_ == 2)
+            drow = (double)L.prim_quartets < 4.0 * (double)L.quartets && lds_drow <= (size_t)max_lds_ / 2;
+        drow = drow && grouped && lds_drow <= (size_t)max_lds_;
         L.drow = drow ? 1 : 0;
         const bool hl = hl_class(L.la, L.lb, L.lc, L.ld);
         L.variant = hl ? 3 : (grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2));

@@ -168,7 +168,7 @@ private:
     int nitems_ = 0, max_nfcd_ = 1;
     bool jslab_whole_ = true;     // one reduction after all launches (else one per launch)
     bool no_jslab_ = false;       // LIBRINT_NO_JSLAB: global atomics only
-    int drow_mode_ = 0;           // LIBRINT_DROW=1: stage the bra density rows in LDS
+    int drow_mode_ = 2;           // bra density rows in LDS: 0 never, 1 where they fit, 2 cheap quartets (LIBRINT_DROW)
     int2 *d_pp_idx_ = nullptr;      // adjoint kernel: primitive indices per record
     int2 *d_shell_slot_ = nullptr;  // {atom, first primitive slot} per shell
     double *d_slot_coef_ = nullptr; // contraction coefficient per slot

@@ -340,6 +340,30 @@ def test_stream_counts_agree(gpu, builder, monkeypatch):
     assert np.abs(out["1m"] - out["1"]).max() < 1e-6
 
 
+def test_density_rows_in_lds_vs_oracle(gpu, monkeypatch):
+    """Uncontracted water-cluster lists stage the bra's density rows in LDS
+    (few primitive quartets per quartet); with and without the staging the
+    build equals the oracle's screened build on the same quartets."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("density rows in LDS, (H2O)4 even-tempered")
+    m = mol.water_cluster(4)
+    P = _rand_sym(m.nao(), 13)
+    Gr, cnt = refcpu.fock2e_screened(m, P, 1e-12)
+    out = {}
+    for mode in ("2", "0"):
+        monkeypatch.setenv("LIBRINT_DROW", mode)
+        ctx = FockContext(m, schwarz_tol=1e-12)
+        out[mode] = ctx.fock(P)
+        assert ctx.stats()["quartets"] == cnt
+        staged = [c for c in ctx.class_stats() if c["drow"]]
+        assert (len(staged) > 0) == (mode == "2")
+        ctx.close()
+    scale = np.abs(Gr).max()
+    assert np.abs(out["2"] - Gr).max() < 1e-13 * scale
+    assert np.abs(out["0"] - Gr).max() < 1e-13 * scale
+
+
 def test_mixed_precision_d_shells(gpu):
     from librint_amd import mol
     from librint_amd.context import FockContext
