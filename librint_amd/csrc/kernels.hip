@@ -1087,12 +1087,25 @@ __global__ __launch_bounds__(256) void jreduce_kernel(const JRedItem *__restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double acc = 0.0;
     int any = 0;
-    for (int k = wave; k < it.lcount; k += 4) {
-        const int t = item_tasks[it.lfirst + k];
-        if (flags && !((accept >> flags[t]) & 1)) continue;
-        if (lane < tasks[t].nket) {
-            acc += J[it.jbase + (long long)(t - it.tfirst) * (64 * nfcd) + e * 64 + lane];
-            any = 1;
+    // wave w takes the item's tasks k = w, w + 4, ...: their indices, flags
+    // and ket counts are loaded at once (lane j holds task w + 4 j of the
+    // current block of 256), so the slab loads below do not wait on them
+    for (int base = wave; base < it.lcount; base += 256) {
+        const int kk = base + 4 * lane;
+        int tl = 0, nkl = 0;
+        if (kk < it.lcount) {
+            tl = item_tasks[it.lfirst + kk];
+            if (!flags || ((accept >> flags[tl]) & 1)) nkl = tasks[tl].nket;
+        }
+        const int cnt = min(64, (it.lcount - base + 3) / 4);
+#pragma unroll 4
+        for (int j = 0; j < cnt; j++) {
+            const int t = __builtin_amdgcn_readlane(tl, j);
+            const int nket = __builtin_amdgcn_readlane(nkl, j);
+            if (lane < nket) {
+                acc += J[it.jbase + (long long)(t - it.tfirst) * (64 * nfcd) + e * 64 + lane];
+                any = 1;
+            }
         }
     }
     part[wave][lane] = acc;
