@@ -85,6 +85,12 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
     // quartet is evaluated, hiding their latency.  Heavier classes keep the
     // plain loop: the extra registers would cost them occupancy.
     constexpr bool PIPE = LA == 0 && LC == 0 && C::NF <= 9;
+    // (01|00) keeps its bra loop rolled: with the occupancy request on its
+    // kernel (jk_group_kernel) it then fits 128 VGPRs with 5 spilled
+    // (4 wavefronts per SIMD instead of 3; measured 0.500 -> 0.471 ms), where
+    // the unrolled loop spills 20.  The other light classes gain from the
+    // unrolled pair.
+    constexpr int BUNROLL = (LA == 0 && LB == 1 && LC == 0 && LD == 0) ? 1 : 2;
     if constexpr (PIPE) {
         PrimPair qn = B.pp[ket.off];
         for (int ik = 0; ik < knpp; ik++) {
@@ -92,7 +98,7 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
             if (ik + 1 < knpp) qn = B.pp[ket.off + ik + 1];
             const T qd[3] = {(T)q.pbx - rbd[0], (T)q.pby - rbd[1], (T)q.pbz - rbd[2]};
             PrimPair pn = B.pp[boff];
-#pragma unroll 2
+#pragma unroll BUNROLL
             for (int ib = 0; ib < bnpp; ib++) {
                 const PrimPair p = pn;
                 if (ib + 1 < bnpp) pn = B.pp[boff + ib + 1];
@@ -343,8 +349,17 @@ __host__ __device__ constexpr int RysLdsDoubles()
 // gmask: per group, bit f set when the group holds a task with flag f.
 // DROW: the bra's density rows (NFA + NFB rows of nao) are staged in LDS
 // beside the exchange rows, so the exchange terms read D from LDS.
+// Minimum wavefronts per SIMD asked of the compiler per class: 4 for (01|00)
+// (its register count would otherwise allow 3), no request elsewhere nor
+// with the density rows in LDS (the LDS then allows two workgroups per CU).
+template <int LA, int LB, int LC, int LD, bool DROW>
+struct GroupWaves {
+    static constexpr int value = (LA == 0 && LB == 1 && LC == 0 && LD == 0 && !DROW) ? 4 : 1;
+};
+
 template <int LA, int LB, int LC, int LD, typename T, bool DROW>
-__global__ __launch_bounds__(256) void jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GroupWaves<LA, LB, LC, LD, DROW>::value))) void
+jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
                                                        const JKTask *__restrict__ tasks, int *__restrict__ counter,
                                                        const unsigned char *__restrict__ flags,
                                                        const unsigned char *__restrict__ gmask, int accept, DevBasis B,
