@@ -350,11 +350,15 @@ __host__ __device__ constexpr int RysLdsDoubles()
 // DROW: the bra's density rows (NFA + NFB rows of nao) are staged in LDS
 // beside the exchange rows, so the exchange terms read D from LDS.
 // Minimum wavefronts per SIMD asked of the compiler per class: 4 for (01|00)
-// (its register count would otherwise allow 3), no request elsewhere nor
+// (153 VGPRs would allow 3; 128 with 5 spilled), 5 for (00|00) (102 -> 96
+// VGPRs, no spill); no request elsewhere (59 spills for 3 in (01|01)) nor
 // with the density rows in LDS (the LDS then allows two workgroups per CU).
 template <int LA, int LB, int LC, int LD, bool DROW>
 struct GroupWaves {
-    static constexpr int value = (LA == 0 && LB == 1 && LC == 0 && LD == 0 && !DROW) ? 4 : 1;
+    static constexpr int value = DROW ? 1
+                                 : (LA == 0 && LB == 1 && LC == 0 && LD == 0)   ? 4
+                                 : (LA == 0 && LB == 0 && LC == 0 && LD == 0)   ? 5
+                                                                                : 1;
 };
 
 template <int LA, int LB, int LC, int LD, typename T, bool DROW>
