@@ -29,8 +29,8 @@ HBM_PEAK = 8000.0         # GB/s
 # and WRITE_SIZE runs scaled by the same counters on a copy of known size
 # (profiles/jk_pmc_traffic_c16h34_631g.md); keyed by (workload, precision, kernel).
 TRAFFIC = {
-    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, double, false>"): (12534.1 * 2.0 + 100074.6) * 1024,
-    ("c16h34_631g", "fp64", "jk_group_kernel<0, 0, 0, 0, double, false>"): (9211.4 * 2.0 + 75912.5) * 1024,
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, double, false>"): (16877.7 * 2.0 + 125931.6) * 1024,
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 0, 0, 0, double, false>"): (9216.0 * 2.0 + 75920.5) * 1024,
 }
 
 
