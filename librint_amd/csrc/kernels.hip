@@ -1117,7 +1117,6 @@ __global__ __launch_bounds__(256) void jreduce_kernel(const JRedItem *__restrict
             if (!flags || ((accept >> flags[tl]) & 1)) nkl = tasks[tl].nket;
         }
         const int cnt = min(64, (it.lcount - base + 3) / 4);
-#pragma unroll 4
         for (int j = 0; j < cnt; j++) {
             const int t = __builtin_amdgcn_readlane(tl, j);
             const int nket = __builtin_amdgcn_readlane(nkl, j);
