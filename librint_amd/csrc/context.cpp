@@ -787,17 +787,7 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     int ns = std::min(nstreams, nl - 1);
     if (!jslab_whole_) ns = std::min(ns, slab_streams_);
     ns = std::max(ns, 1);
-    if (ns > 1 && (int)side_.size() < ns - 1) {
-        while ((int)side_.size() < ns - 1) {
-            hipStream_t q = nullptr;
-            check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "hipStreamCreate");
-            side_.push_back(q);
-            hipEvent_t e = nullptr;
-            check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            evj_.push_back(e);
-        }
-        if (!evf_) check(hipEventCreateWithFlags(&evf_, hipEventDisableTiming), "hipEventCreate");
-    }
+    if (ns > 1) ensure_side_streams(ns - 1);
     for (int j = 0; j < nl; j++) {
         const int k = j == 0 ? dominant_ : (j <= dominant_ ? j - 1 : j);
         ClassLaunch &L = launches_[k];
@@ -860,6 +850,21 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
               "coulomb reduction launch");
     check(hipEventRecord(ev1_, st), "hipEventRecord");
     check(launch_symmetrize(d_M_, G_dev, nao_, st), "symmetrize launch");
+}
+
+// At least n extra streams (default priority, non-blocking) with one join
+// event each, and the fork event.
+void Context::ensure_side_streams(int n)
+{
+    while ((int)side_.size() < n) {
+        hipStream_t q = nullptr;
+        check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "hipStreamCreate");
+        side_.push_back(q);
+        hipEvent_t e = nullptr;
+        check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        evj_.push_back(e);
+    }
+    if (n > 0 && !evf_) check(hipEventCreateWithFlags(&evf_, hipEventDisableTiming), "hipEventCreate");
 }
 
 void Context::fock_host(const double *D, double *G)
@@ -1131,15 +1136,32 @@ void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
     check(hipMalloc(&dg, sizeof(double) * nout), "hipMalloc");
     check(hipMemset(dg, 0, sizeof(double) * nout), "hipMemset");
     const DevBasis B = devb();
+    // The class launches of both sides are dealt round-robin over the null
+    // stream and nst - 1 extra streams (LIBRINT_GRAD_STREAMS, default 4): they
+    // share only the atomically updated outputs, and the small launches and
+    // launch tails overlap.
+    int nst = 4;
+    if (const char *e = getenv("LIBRINT_GRAD_STREAMS")) nst = std::max(1, std::min(8, atoi(e)));
+    ensure_side_streams(nst - 1);
     check(hipEventRecord(ev0_, 0), "hipEventRecord");
+    if (nst > 1) {
+        check(hipEventRecord(evf_, 0), "hipEventRecord");
+        for (int i = 0; i < nst - 1; i++) check(hipStreamWaitEvent(side_[i], evf_, 0), "hipStreamWaitEvent");
+    }
+    int k = 0;
+    auto stream_of = [&](int j) { return (j % nst) == 0 ? (hipStream_t)0 : side_[(j % nst) - 1]; };
     for (const ClassLaunch &L : launches_)
         check(launch_grad(L.la, L.lb, L.lc, L.ld, 0, d_tasks_ + L.first, L.count, B, d_pp_idx_, d_shell_slot_,
-                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, 0),
+                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, stream_of(k++)),
               "grad launch");
     for (const GradLaunch &G : glaunch1_)
         check(launch_grad(G.la, G.lb, G.lc, G.ld, 1, d_gtasks1_ + G.first, G.count, B, d_pp_idx_, d_shell_slot_,
-                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, 0),
+                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, stream_of(k++)),
               "grad launch");
+    for (int i = 0; i < nst - 1; i++) {
+        check(hipEventRecord(evj_[i], side_[i]), "hipEventRecord");
+        check(hipStreamWaitEvent(0, evj_[i], 0), "hipStreamWaitEvent");
+    }
     check(hipEventRecord(ev1_, 0), "hipEventRecord");
     std::vector<double> h(nout);
     check(hipMemcpy(h.data(), dg, sizeof(double) * nout, hipMemcpyDeviceToHost), "hipMemcpy D2H");

@@ -111,6 +111,7 @@ private:
     void compute_schwarz();
     void build_tasks();
     void upload();
+    void ensure_side_streams(int n);
 
     std::vector<int> atm_, bas_;
     std::vector<double> env_;
