@@ -1,15 +1,20 @@
 """Benchmark: RHF Fock build (fused Rys ERI + J/K) on one MI355X.
 
-Workload (default): n-hexadecane C16H34, all-trans, 6-31G (148 shells, 212
-cartesian functions), Schwarz threshold 1e-12, FP64.  One step = one full
-two-electron Fock build G = J - K/2 from a density already resident in HBM
-(core-Hamiltonian guess density), i.e. every surviving canonical shell
-quartet's ERIs are recomputed and contracted.
+Workload (default): a seeded cluster of 32 water molecules in a 10 A cube
+with the even-tempered s,p,d basis (512 shells, 992 cartesian functions, up
+to 5 Rys roots), Schwarz threshold 1e-12.  One step = one full two-electron
+Fock build G = J - K/2 from a density already resident in HBM
+(core-Hamiltonian guess density): every surviving canonical shell quartet's
+ERIs are recomputed and contracted.  The headline value is the FP64 build;
+`mixed_vs_fp64` times the same build with FP32 recurrences for the tasks
+whose Schwarz bound times the largest density block they read is below
+tau_mix (1e-8 .. 1e-5), with the energy error against the FP64 build.
+n-hexadecane C16H34/6-31G is timed the same way under `extra_workloads`.
 
 Prints one JSON line (metric, value = canonical surviving ERI quartets per
 second, ms_per_step = Fock-build time per iteration, roofline of the
-dominant kernel from HIP events, CPU baseline from the oracle on a bounded
-sample).
+dominant kernel from HIP events, CPU baselines from the oracle on bounded
+samples).
 """
 import argparse
 import json
@@ -168,7 +173,8 @@ def bench_grad(args):
     stencil, steps 1e-4 bohr and 1e-4*max(zeta, 1) in the exponents; the
     3-point quotient at the same step is reported beside it)."""
     from librint_amd import autograd
-    name = args.workload if args.workload != "c16h34_631g" else "w8_et"
+    # the gradient workload is the 8-water cluster unless another one is named
+    name = "w8_et" if args.workload == "w32_et" else args.workload
     m, _ = workload(name)
     tau = 1e-14
     E0, P0, it, ctx = autograd.rhf(m, conv=1e-10, schwarz_tol=tau)
@@ -223,20 +229,113 @@ def bench_grad(args):
     print(json.dumps(line), flush=True)
 
 
+def timed_builds(ctx, Pd, Gd, steps, warmup):
+    """Wall ms per Fock build over `steps` builds after `warmup`, bracketed by
+    device synchronisation."""
+    import torch
+    for _ in range(warmup):
+        ctx.fock_device(Pd, Gd)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.fock_device(Pd, Gd)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def fock_line(name, steps, warmup, tau_mix_list, traffic_precision="fp64"):
+    """FP64 Fock-build timing of one workload (core-Hamiltonian density),
+    per-class device times from the library's HIP events, the roofline of the
+    dominant launch and, for each tau_mix, the mixed-precision build timed the
+    same way with its energy error against the FP64 GPU build.  Returns the
+    result dict and (molecule, density, context, FP64 G, wall ms)."""
+    import torch
+    from librint_amd.context import FockContext
+    m, tau = workload(name)
+    P = core_guess_density(m)
+    ctx = FockContext(m, schwarz_tol=tau)
+    dev = torch.device("cuda:0")
+    Pd = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
+    Gd = torch.empty_like(Pd)
+    ms_per_step = timed_builds(ctx, Pd, Gd, steps, warmup)
+    # device time of the J/K kernels (HIP events recorded by the library on
+    # its streams around the class launches), more builds timed one by one
+    ev_ms, cs_runs = [], []
+    for _ in range(min(steps, 10)):
+        ctx.fock(P)
+        ev_ms.append(ctx.stats()["kernel_ms"])
+        cs_runs.append(ctx.class_stats())
+    st = ctx.stats()
+    cstats = cs_runs[-1]
+    for k, c in enumerate(cstats):
+        c["ms"] = float(np.mean([r[k]["ms"] for r in cs_runs]))
+    classes = [{"class": "(%d%d|%d%d)" % (int(c["la"]), int(c["lb"]), int(c["lc"]), int(c["ld"])),
+                "kernel": c["kernel"], "quartets": int(c["quartets"]), "prim_quartets": int(c["prim_quartets"]),
+                "lane_util": c["quartets"] / max(c["lanes"], 1), "ms": c["ms"],
+                "tflops": c["flops"] / max(c["ms"], 1e-9) / 1e9} for c in cstats]
+    jk_ms = float(np.mean(ev_ms))
+    # dominant kernel: the class launch with the most algorithmic flops.  The
+    # library runs it first and alone, so its event time is its own; the
+    # other launches then overlap on several streams and their times include
+    # the sharing.
+    dom = max(cstats, key=lambda c: c["flops"])
+    dom_achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
+    achieved_all = st["flops"] / (jk_ms * 1e-3) / 1e12
+    quartets = st["quartets"]
+    line = {
+        "value": quartets / (ms_per_step * 1e-3),
+        "ms_per_step": ms_per_step,
+        "fock_s_per_iter": ms_per_step * 1e-3,
+        "integrals_per_s": st["integrals"] / (ms_per_step * 1e-3),
+        "prim_quartets_per_s": st["prim_quartets"] / (ms_per_step * 1e-3),
+        "config": {"workload": name, "nbas": m.nbas, "nao": m.nao(), "schwarz": tau,
+                   "precision": "fp64", "quartets": int(quartets), "prim_quartets": int(st["prim_quartets"]),
+                   "integrals": int(st["integrals"]), "candidates": int(st["candidates"]),
+                   "launches": int(st["launches"]),
+                   "streams": int(os.environ.get("LIBRINT_STREAMS", "2"))},
+        "roofline": {"bound": "valu_fp64", "achieved": dom_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": dom_achieved / FP64_PEAK_TFLOPS,
+                     "traffic": TRAFFIC.get((name, traffic_precision, dom["kernel"])),
+                     "kernel": dom["kernel"], "kernel_ms": dom["ms"], "flops_per_launch": dom["flops"],
+                     "all_classes": {"achieved": achieved_all, "frac": achieved_all / FP64_PEAK_TFLOPS,
+                                     "ms": jk_ms, "flops": st["flops"]}},
+        "classes": classes,
+    }
+    G64 = ctx.fock(P)
+    mixed = []
+    for tau_mix in tau_mix_list:
+        # FP32 recurrences for the 64-ket tasks whose largest
+        # Q_ab Q_cd max|D_block| is below tau_mix, decided per build
+        ctx.set_precision(1, tau_mix, 0.0)
+        ms_mixed = timed_builds(ctx, Pd, Gd, steps, warmup)
+        Gm = ctx.fock(P)
+        fc = ctx.flag_counts()
+        q64, q32 = fc["quartets"][:2]
+        mixed.append({
+            "tau_mix": tau_mix, "ms_per_step": ms_mixed, "fp64_ms_per_step": ms_per_step,
+            "speedup": ms_per_step / ms_mixed,
+            "energy_error_vs_fp64_gpu_Eh": float(0.5 * np.sum(P * (Gm - G64))),
+            "max_abs_dG": float(np.abs(Gm - G64).max()),
+            "quartets_fp64_fp32": [q64, q32], "fp32_quartet_share": q32 / max(q64 + q32, 1)})
+    if tau_mix_list:
+        ctx.set_precision(0, 0.0, 0.0)
+        line["mixed_vs_fp64"] = mixed
+    return line, (m, P, ctx, G64, ms_per_step)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c16h34_631g")
+    ap.add_argument("--workload", default="w32_et")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
-    ap.add_argument("--tau-mix", type=float, default=1e-6)
-    ap.add_argument("--tau-skip", type=float, default=0.0)
-    ap.add_argument("--tau-mix-compare", type=float, nargs="*", default=[1e-5, 1e-3],
-                    help="FP64 runs also time the mixed-precision build at these thresholds")
+    ap.add_argument("--tau-mix-compare", type=float, nargs="*", default=[1e-8, 1e-7, 1e-6, 1e-5],
+                    help="also time the mixed-precision build at these thresholds")
     ap.add_argument("--no-mixed-compare", action="store_true")
+    ap.add_argument("--extra", nargs="*", default=["c16h34_631g"],
+                    help="further workloads timed the same way (FP64 and mixed), reported under 'extra_workloads'")
     ap.add_argument("--shares", type=int, default=1,
                     help="time each device's share of a G-device split of the Fock build on this one GPU")
     ap.add_argument("--grad", action="store_true",
@@ -267,138 +366,60 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "copies": max(args.steps, 1)}))
         return
-    m, tau = workload(args.workload)
-    P = core_guess_density(m)
-    ctx = FockContext(m, schwarz_tol=tau)
     if args.shares > 1:
-        return bench_shares(args, m, P, ctx)
-    mixed = args.precision == "mixed"
-    G64 = None
-    if mixed or args.tau_skip > 0:
-        G64 = ctx.fock(P)  # FP64 reference build (untimed) for the error report
-        ctx.set_precision(1 if mixed else 0, args.tau_mix if mixed else 0.0, args.tau_skip)
-    dev = torch.device("cuda:0")
-    Pd = torch.from_numpy(np.ascontiguousarray(P)).to(dev)
-    Gd = torch.empty_like(Pd)
-    for _ in range(args.warmup):
-        ctx.fock_device(Pd, Gd)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.fock_device(Pd, Gd)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    # device time of the J/K kernels (HIP events recorded by the library on
-    # its stream around the class launches), one more timed build each
-    ev_ms, cs_runs = [], []
-    for _ in range(min(args.steps, 10)):
-        ctx.fock(P)
-        ev_ms.append(ctx.stats()["kernel_ms"])
-        cs_runs.append(ctx.class_stats())
-    st = ctx.stats()
-    err = None
-    if G64 is not None:
-        Gm = ctx.fock(P)
-        fc = ctx.flag_counts()
-        err = {"tau_mix": args.tau_mix if mixed else None, "tau_skip": args.tau_skip,
-               "energy_error_Eh": float(0.5 * np.sum(P * (Gm - G64))),
-               "max_abs_dG": float(np.abs(Gm - G64).max()),
-               "quartets_fp64_fp32_skipped": fc["quartets"],
-               "prim_quartets_fp64_fp32_skipped": fc["prim_quartets"]}
-    # per-launch device times from the library's HIP events, averaged over
-    # the event-timed builds
-    cstats = cs_runs[-1]
-    for k, c in enumerate(cstats):
-        c["ms"] = float(np.mean([r[k]["ms"] for r in cs_runs]))
-    classes = [{"class": "(%d%d|%d%d)" % (int(c["la"]), int(c["lb"]), int(c["lc"]), int(c["ld"])),
-                "kernel": c["kernel"], "quartets": int(c["quartets"]), "prim_quartets": int(c["prim_quartets"]),
-                "lane_util": c["quartets"] / max(c["lanes"], 1), "ms": c["ms"],
-                "tflops": c["flops"] / max(c["ms"], 1e-9) / 1e9} for c in cstats]
-    ms_per_step = wall / args.steps * 1e3
-    quartets = st["quartets"]
-    jk_ms = float(np.mean(ev_ms))
-    # dominant kernel: the class launch with the most algorithmic flops.  The
-    # library runs it first and alone, so its event time is its own; the
-    # other launches then overlap on several streams and their times include
-    # the sharing.
-    dom = max(cstats, key=lambda c: c["flops"])
-    dom_achieved = dom["flops"] / (dom["ms"] * 1e-3) / 1e12
-    achieved_all = st["flops"] / (jk_ms * 1e-3) / 1e12
+        m, tau = workload(args.workload)
+        P = core_guess_density(m)
+        return bench_shares(args, m, P, FockContext(m, schwarz_tol=tau))
+    taus = [] if args.no_mixed_compare else args.tau_mix_compare
+    body, (m, P, ctx, G64, ms_per_step) = fock_line(args.workload, args.steps, args.warmup, taus)
+    quartets = body["config"]["quartets"]
     line = {
-        "metric": "RHF Fock-build s/iter + ERI quartets/s (FP64 fused J/K)",
-        "value": quartets / (ms_per_step * 1e-3),
+        "metric": "RHF Fock-build s/iter + ERI quartets/s, mixed vs FP64 at <=1 uEh error",
+        "value": body.pop("value"),
         "unit": "quartets/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "fock_s_per_iter": ms_per_step * 1e-3,
-        "integrals_per_s": st["integrals"] / (ms_per_step * 1e-3),
-        "prim_quartets_per_s": st["prim_quartets"] / (ms_per_step * 1e-3),
+        "ms_per_step": body.pop("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic geometry (all-trans zig-zag, ideal angles), core-Hamiltonian density",
-        "config": {"workload": args.workload, "nbas": m.nbas, "nao": m.nao(), "schwarz": tau,
-                   "precision": "fp64", "quartets": int(quartets), "prim_quartets": int(st["prim_quartets"]),
-                   "integrals": int(st["integrals"]), "candidates": int(st["candidates"]),
-                   "launches": int(st["launches"]),
-                   "streams": int(os.environ.get("LIBRINT_STREAMS", "4" if args.precision == "mixed" else "2"))},
-        "roofline": {"bound": "mfma", "achieved": dom_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": dom_achieved / FP64_PEAK_TFLOPS,
-                     "traffic": TRAFFIC.get((args.workload, args.precision, dom["kernel"])),
-                     "kernel": dom["kernel"], "kernel_ms": dom["ms"], "flops_per_launch": dom["flops"],
-                     "all_classes": {"achieved": achieved_all, "frac": achieved_all / FP64_PEAK_TFLOPS,
-                                     "ms": jk_ms, "flops": st["flops"]}},
-        "classes": classes,
+        "data": "synthetic seeded geometry, core-Hamiltonian density; value = FP64 fused J/K build, "
+                "mixed_vs_fp64 = the same build with FP32 recurrences below tau_mix",
     }
-    if err is None and not args.no_mixed_compare:
-        # the same build with FP32 recurrences for the quartets whose
-        # Q_ab Q_cd max|D| < tau_mix, timed the same way, against this FP64 G
-        G64 = ctx.fock(P)
-        line["mixed_vs_fp64"] = []
-        for tau_mix in args.tau_mix_compare:
-            ctx.set_precision(1, tau_mix, 0.0)
-            for _ in range(args.warmup):
-                ctx.fock_device(Pd, Gd)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(args.steps):
-                ctx.fock_device(Pd, Gd)
-            torch.cuda.synchronize()
-            ms_mixed = (time.perf_counter() - t0) / args.steps * 1e3
-            Gm = ctx.fock(P)
-            fc = ctx.flag_counts()
-            line["mixed_vs_fp64"].append({
-                "tau_mix": tau_mix, "ms_per_step": ms_mixed, "fp64_ms_per_step": ms_per_step,
-                "speedup": ms_per_step / ms_mixed,
-                "energy_error_Eh": float(0.5 * np.sum(P * (Gm - G64))), "max_abs_dG": float(np.abs(Gm - G64).max()),
-                "quartets_fp64_fp32": fc["quartets"][:2]})
-        ctx.set_precision(0, 0.0, 0.0)
-    if err is not None:
-        line["metric"] = "RHF Fock-build s/iter + ERI quartets/s (%s fused J/K)" % ("mixed FP32/FP64" if mixed else "FP64, density screened")
-        line["dtype"] = "f32+f64" if mixed else "f64"
-        line["config"]["precision"] = args.precision
-        line["mixed"] = err
+    line.update(body)
+    ctx.close()
     if not args.no_cpu:
         rate, t_full, nq, used = cpu_baseline(m, P, args.cpu_seconds)
         line["cpu_baseline"] = {
             "value": rate, "unit": "quartets/s", "cores": 1, "kind": "port",
             "sample": "%d random ordered shell quartets of the nbas^4 reference loop (%.1f s, oracle "
-                      "integral2e_fock restatement, 1 thread)" % (nq, used),
+                      "integral2e_fock restatement: no symmetry, no screening, FP64, 1 thread)" % (nq, used),
             "fock_s_per_iter_extrapolated": t_full,
-            "gpu_speedup_fock": t_full / (ms_per_step * 1e-3),
         }
-        trate, nth, tq, tused, phases = cpu_threaded(m, P, tau, args.cpu_seconds)
+        stride = max(16, int(quartets / 2e6))
+        trate, nth, tq, tused, phases = cpu_threaded(m, P, line["config"]["schwarz"], args.cpu_seconds, stride)
         line["cpu_threaded"] = {
             "value": trate, "unit": "quartets/s", "cores": nth, "kind": "port",
             "cpu": cpu_model(),
-            "sample": "%d of the %d Schwarz-surviving canonical quartets (bra pairs ij %% 16 in 0..%d, %.1f s; "
-                      "oracle screened 8-fold-symmetric Fock build, OpenMP)" % (tq, quartets, phases - 1, tused),
+            "sample": "%d of the %d Schwarz-surviving canonical quartets (bra pairs ij %% %d in 0..%d, %.1f s; "
+                      "oracle screened 8-fold-symmetric Fock build, OpenMP)" % (tq, quartets, stride, phases - 1,
+                                                                                 tused),
             "fock_s_per_iter_extrapolated": quartets / trate,
-            "gpu_speedup_fock": quartets / trate / (ms_per_step * 1e-3),
+            "gpu_speedup_fock": quartets / trate / (line["ms_per_step"] * 1e-3),
         }
+    extra = {}
+    for name in args.extra:
+        if name == args.workload:
+            continue
+        b, (_, _, c2, _, _) = fock_line(name, args.steps, args.warmup, [] if args.no_mixed_compare else
+                                        [1e-8, 1e-6, 1e-5])
+        c2.close()
+        b.pop("classes")
+        extra[name] = b
+    if extra:
+        line["extra_workloads"] = extra
     print(json.dumps(line), flush=True)
 
 

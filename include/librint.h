@@ -110,13 +110,13 @@ int librint_ctx_set_schwarz(librint_ctx *ctx, double tol);
 int librint_ctx_set_shard(librint_ctx *ctx, int nshards, int shard);
 
 /* Precision of the Fock build.  mode 0: FP64 everywhere (default).
- * mode 1: mixed -- quartets with Q_ab*Q_cd*dmax < tau_mix run their Rys
- * recurrences in FP32 (the Rys table too; J/K accumulation stays FP64).  The
- * split is fixed when the quartet lists are (re)built; dmax <= 0 takes max|D|
- * of the next density.  tau_skip > 0 additionally drops, per build, the 64-ket
- * tasks whose largest bound Q_ab*Q_cd*max(|D| over the six blocks a quartet
- * reads) is below tau_skip (density screening; also honoured in mode 0).
- * Returns 0 on success. */
+ * mode 1: mixed -- every build bounds each 64-ket task by the largest
+ * Q_ab*Q_cd*max(|D| over the six density blocks a quartet reads) of its kets,
+ * from that build's density; tasks below tau_mix run their Rys recurrences in
+ * FP32 (the Rys table too; J/K accumulation stays FP64).  tau_skip > 0
+ * additionally drops, per build, the tasks below tau_skip (density
+ * screening; also honoured in mode 0).  dmax is ignored (kept for ABI
+ * compatibility).  Returns 0 on success. */
 int librint_ctx_set_precision(librint_ctx *ctx, int mode, double tau_mix, double tau_skip, double dmax);
 /* Quartets and primitive quartets of the last build by task flag:
  * out[0..2] quartets FP64/FP32/skipped, out[3..5] primitive quartets. */
@@ -145,13 +145,16 @@ int librint_schwarz(librint_ctx *ctx, double *out);
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 
-/* Per angular-class launch of the current plan, 13 values each:
+/* Per angular-class launch of the current plan, 14 values each:
  * la, lb, lc, ld, quartets, primitive quartets, algorithmic flops, lanes
  * launched (64 x tasks), device ms of the launch in the last host-side build
  * (librint_fock), task count, FP32 recurrences (0/1), kernel of the last
  * build (0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel for
  * classes with an f or g shell, -1 none yet),
- * bra density rows staged in LDS (0/1).
+ * bra density rows staged in LDS (0/1), task flags accepted (bit 0: FP64
+ * tasks, bit 1: FP32 tasks; in mixed precision a class with FP32 kernels has
+ * an FP64 launch (1) and an FP32 twin (2) over the same tasks, the counts
+ * being those of the whole class in both).
  * Returns the number of launches (out may be NULL to query it); writes at
  * most n launches. */
 int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);

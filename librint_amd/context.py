@@ -40,9 +40,11 @@ class FockContext:
         if library.librint_ctx_set_schwarz(self.h, float(tol)) != 0:
             raise RuntimeError(utils.last_error())
 
-    def set_precision(self, mode, tau_mix=0.0, tau_skip=0.0, dmax=0.0):
-        """mode 0 FP64; mode 1 mixed (FP32 recurrences for Q Q dmax < tau_mix)."""
-        if library.librint_ctx_set_precision(self.h, int(mode), float(tau_mix), float(tau_skip), float(dmax)) != 0:
+    def set_precision(self, mode, tau_mix=0.0, tau_skip=0.0):
+        """mode 0 FP64; mode 1 mixed: per build, the 64-ket tasks whose
+        largest Q_ab Q_cd max|D_block| is below tau_mix run FP32 recurrences;
+        tau_skip > 0 drops the tasks below it (both modes)."""
+        if library.librint_ctx_set_precision(self.h, int(mode), float(tau_mix), float(tau_skip), 0.0) != 0:
             raise RuntimeError(utils.last_error())
 
     def flag_counts(self):
@@ -99,11 +101,11 @@ class FockContext:
     def class_stats(self):
         """Per angular-class launch: counts, flops and the last build's device ms."""
         n = library.librint_ctx_class_stats(self.h, None, 0)
-        out = np.zeros(13 * max(n, 1))
+        out = np.zeros(14 * max(n, 1))
         library.librint_ctx_class_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
         keys = ("la", "lb", "lc", "ld", "quartets", "prim_quartets", "flops", "lanes", "ms", "tasks", "fp32",
-                "variant", "drow")
-        rows = [dict(zip(keys, out[13 * k:13 * k + 13].tolist())) for k in range(n)]
+                "variant", "drow", "accept")
+        rows = [dict(zip(keys, out[14 * k:14 * k + 14].tolist())) for k in range(n)]
         for r in rows:
             v = int(r["variant"])
             ls = "%d, %d, %d, %d" % (r["la"], r["lb"], r["lc"], r["ld"])

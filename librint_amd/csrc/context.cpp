@@ -136,7 +136,6 @@ Context::~Context()
     dev_free(d_gmask_w_);
     dev_free(d_gmask_);
     dev_free(d_dsh_);
-    dev_free(d_dmax_);
     dev_free(d_counters_);
     dev_free(d_M_);
     dev_free(d_D_);
@@ -419,8 +418,11 @@ void Context::build_tasks()
         std::vector<std::pair<double, JKGroup>> groups;
         ClassLaunch stat;
     };
-    std::map<std::tuple<int, int, int>, Acc> acc;  // (bra class, ket class, fp32)
-    const bool mixed = precision_ == 1 && dmax_ > 0.0 && tau_mix_ > 0.0;
+    std::map<std::tuple<int, int, int>, Acc> acc;  // (bra class, ket class, 0)
+    // mixed precision: the classes with FP32 kernels get a second launch over
+    // the same tasks (see below); which of the two evaluates a task is decided
+    // per build from that build's density
+    const bool mixed = precision_ == 1 && tau_mix_ > 0.0;
     for (int X = 0; X < nb; X++) {
         const Bin &bx = bins_[X];
         int la, lb;
@@ -446,23 +448,9 @@ void Context::build_tasks()
                 if (n <= 0) continue;
                 const double fp = flops_prim(la, lb, lc, ld);
                 const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
-                // mixed: kets [0, n64) keep FP64, [n64, n) have Q_b Q_k dmax < tau_mix
-                int n64 = n;
-                if (mixed && class_has_fp32(la, lb, lc, ld)) {
-                    int l2 = 0, h2 = n;
-                    while (l2 < h2) {
-                        const int mid = (l2 + h2) / 2;
-                        if (Qp * pairs_[by.first + mid].Q * dmax_ >= tau_mix_) l2 = mid + 1;
-                        else h2 = mid;
-                    }
-                    // round the split up to a whole 64-ket window (those
-                    // kets stay FP64) so every task starts on a window
-                    n64 = std::min(n, (l2 + 63) / 64 * 64);
-                }
-                for (int part = 0; part < 2; part++) {
-                    const int k_begin = part ? n64 : 0, k_end = part ? n : n64;
+                {
+                    const int part = 0, k_begin = 0, k_end = n;
                     const int nn = k_end - k_begin;
-                    if (nn <= 0) continue;
                     const double fl = (double)nn * bx.npp * by.npp * fp + 12.0 * nf * nn;
                     stats_.quartets += nn;
                     stats_.prim_quartets += (long long)nn * bx.npp * by.npp;
@@ -597,6 +585,7 @@ void Context::build_tasks()
         }
         L.count = (int)tasks.size() - L.first;
         L.gcount = (int)groups.size() - L.gfirst;
+        L.accept = mixed && class_has_fp32(la, lb, lc, ld) ? 1 : 3;
         if (L.count) launches_.push_back(L);
     }
     stats_.launches = (int)launches_.size();
@@ -656,6 +645,16 @@ void Context::build_tasks()
         L.icount = (int)items.size() - L.ifirst;
     }
     nitems_ = (int)items.size();
+    // the FP32 twins: same tasks, groups, slabs and reduction items
+    for (size_t k = 0, nl = launches_.size(); k < nl; k++)
+        if (launches_[k].accept == 1) {
+            ClassLaunch T = launches_[k];
+            T.fp32 = 1;
+            T.accept = 2;
+            T.twin = 1;
+            launches_.push_back(T);
+        }
+    stats_.launches = (int)launches_.size();
     dev_free(d_items_);
     dev_free(d_item_tasks_);
     d_items_ = dev_upload(items);
@@ -702,15 +701,16 @@ void Context::build_tasks()
     check(hipMemset(d_counters_, 0, sizeof(int) * 2 * std::max<size_t>(launches_.size(), 1)), "hipMemset");
 }
 
-void Context::set_precision(int mode, double tau_mix, double tau_skip, double dmax)
+void Context::set_precision(int mode, double tau_mix, double tau_skip)
 {
     if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
-    if (!(tau_mix >= 0) || !(tau_skip >= 0) || !(dmax >= 0)) throw std::invalid_argument("thresholds must be >= 0");
-    const bool rebuild = mode != precision_ || tau_mix != tau_mix_ || dmax != dmax_;
+    if (!(tau_mix >= 0) || !(tau_skip >= 0)) throw std::invalid_argument("thresholds must be >= 0");
+    // the lists change only when the FP32 twins appear or go; a new
+    // threshold is applied by the next build's classification
+    const bool rebuild = (mode == 1 && tau_mix > 0.0) != (precision_ == 1 && tau_mix_ > 0.0);
     precision_ = mode;
     tau_mix_ = tau_mix;
     tau_skip_ = tau_skip;
-    dmax_ = dmax;
     if (rebuild) build_tasks();
 }
 
@@ -720,34 +720,23 @@ void Context::flag_counts(long long *q3, long long *p3)
     std::vector<unsigned char> fl(h_tasks_.size(), 0);
     if (classified_ && !fl.empty())
         check(hipMemcpy(fl.data(), d_flags_, fl.size(), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-    for (const ClassLaunch &L : launches_)
+    for (const ClassLaunch &L : launches_) {
+        if (L.twin) continue;  // counted with its FP64 launch
         for (int t = L.first; t < L.first + L.count; t++) {
             const JKTask &T = h_tasks_[t];
-            const int f = fl[t] == 2 ? 2 : L.fp32;
+            // flag 1 is evaluated in FP32 only where the class has an FP32 twin
+            const int f = fl[t] == 2 ? 2 : (L.accept == 1 ? fl[t] : 0);
             int b0, b1;
             task_prims(T, pairs_[T.bra].npp, b0, b1);
             if (b0 == 0) q3[f] += T.nket;  // a split quartet counts once
             p3[f] += (long long)T.nket * (b1 - b0) * pairs_[T.ket0].npp;
         }
+    }
 }
 
 void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
 {
     const size_t nn = (size_t)nao_ * nao_;
-    // Mixed precision without a given dmax: take max|D| of this density and
-    // rebuild the quartet lists once.  This must come before anything below
-    // uses the task, group or counter buffers, which the rebuild reallocates.
-    if (precision_ == 1 && tau_mix_ > 0.0 && dmax_ <= 0.0) {
-        // split threshold from this density: max|D|, then rebuild the lists once
-        if (!d_dmax_) check(hipMalloc(&d_dmax_, sizeof(double)), "hipMalloc");
-        check(launch_absmax(D_dev, nn, d_dmax_, st), "absmax launch");
-        double dm = 0.0;
-        check(hipMemcpyAsync(&dm, d_dmax_, sizeof(double), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-        check(hipStreamSynchronize(st), "hipStreamSynchronize");
-        dmax_ = dm > 0.0 ? dm : 1.0;
-        build_tasks();
-    }
-
     if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
     check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
     const DevBasis B = devb();
@@ -758,12 +747,15 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         check(hipDeviceGetAttribute(&max_lds_, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
         if (getenv("LIBRINT_DEBUG")) fprintf(stderr, "librint: max LDS per block %d\n", max_lds_);
     }
-    const bool classify = tau_skip_ > 0.0;
+    // Per-build task flags from the density: skipped below tau_skip, FP32
+    // below tau_mix where the class has an FP32 twin launch
+    const bool mixed = precision_ == 1 && tau_mix_ > 0.0;
+    const bool classify = tau_skip_ > 0.0 || mixed;
     const int nsh = (int)shells_.size();
     if (classify) {
         if (!d_dsh_) check(hipMalloc(&d_dsh_, sizeof(double) * (size_t)nsh * nsh), "hipMalloc");
         check(launch_classify(d_tasks_, (int)h_tasks_.size(), d_task_group_, ngroups_, B, D_dev, d_ao_off_, nsh, nao_,
-                              d_dsh_, 0.0, tau_skip_, d_flags_, d_gmask_w_, d_gmask_, st),
+                              d_dsh_, mixed ? tau_mix_ : 0.0, tau_skip_, d_flags_, d_gmask_w_, d_gmask_, st),
               "classify launch");
     }
     classified_ = classify;
@@ -823,20 +815,20 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         double *js = L.jbase >= 0 ? jbuf + L.jbase : nullptr;
         if (hl) {
             const HLShape S = hl_shape(L.la, L.lb, L.lc, L.ld);
-            check(launch_hl_jk(S, d_tasks_ + L.first, L.count, flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_,
+            check(launch_hl_jk(S, d_tasks_ + L.first, L.count, flags ? flags + L.first : nullptr, L.accept, B, D_dev, d_M_,
                                nao_, hl_scratch(S), nhlscr_, sk),
                   "jk launch (l > 2)");
         } else if (grouped)
             check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
-                                  gmask ? gmask + L.gfirst : nullptr, 1, B, D_dev, d_M_, nao_, js, L.first, drow,
+                                  gmask ? gmask + L.gfirst : nullptr, L.accept, B, D_dev, d_M_, nao_, js, L.first, drow,
                                   sk),
                   "jk group launch");
         else
             check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + L.first, L.count, ctr,
-                            flags ? flags + L.first : nullptr, 1, B, D_dev, d_M_, nao_, js, 0, sk),
+                            flags ? flags + L.first : nullptr, L.accept, B, D_dev, d_M_, nao_, js, 0, sk),
                   "jk launch");
         if (!jslab_whole_ && L.icount)
-            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, 1, jbuf, B,
+            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, L.accept, jbuf, B,
                                  d_M_, nao_, sk),
                   "coulomb reduction launch");
         check(hipEventRecord(evc_[k], sk), "hipEventRecord");
@@ -846,7 +838,7 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         check(hipStreamWaitEvent(st, evj_[i], 0), "hipStreamWaitEvent");
     }
     if (jslab_whole_ && nitems_)
-        check(launch_jreduce(d_items_, nitems_, max_nfcd_, d_item_tasks_, d_tasks_, flags, 1, d_J_, B, d_M_, nao_, st),
+        check(launch_jreduce(d_items_, nitems_, max_nfcd_, d_item_tasks_, d_tasks_, flags, 3, d_J_, B, d_M_, nao_, st),
               "coulomb reduction launch");
     check(hipEventRecord(ev1_, st), "hipEventRecord");
     check(launch_symmetrize(d_M_, G_dev, nao_, st), "symmetrize launch");
@@ -934,16 +926,22 @@ void Context::tensor_device(double *dR)
 
 size_t Context::shells_host(const int *quartets, int nq, double *out)
 {
-    // Each requested (I,J,K,L) maps to a canonical stored pair of pairs; the
-    // block is computed in stored orientation and permuted back here.
+    // A generally contracted input shell is a run of nctr virtual segmented
+    // shells.  Each requested (I,J,K,L) expands into one sub-block per
+    // contraction combination (c0,c1,c2,c3); each sub-block maps to a
+    // canonical stored pair of pairs, is computed in stored orientation and
+    // permuted into the requested block at I = f + nf*c (cint2e_cart layout,
+    // reference src/cint2e.rs:3128-3152).
     struct Req {
         int bra, ket;
         bool swap_bk;
         long long off_dev;
         long long off_out;
-        int dims[4];
+        int vs[4];   // virtual shells of the sub-block
+        int nf[4];   // cartesian functions per virtual shell
+        int ctr[4];  // contraction index of each slot
+        int dims[4]; // full block dims (nf * nctr)
     };
-    const int nb = nbas();
     std::vector<int> cum(bas_.size() / 8 + 1, 0);  // first virtual shell of each input shell
     {
         int v = 0;
@@ -953,7 +951,10 @@ size_t Context::shells_host(const int *quartets, int nq, double *out)
         }
         cum[bas_.size() / 8] = v;
     }
-    (void)nb;
+    auto pidx = [&](int x, int y) {
+        if (x < y) std::swap(x, y);
+        return pair_of_ij_[(size_t)x * (x + 1) / 2 + y];
+    };
     std::vector<Req> reqs;
     std::map<std::pair<int, int>, std::vector<ListTask>> by_cls;
     long long dev_total = 0, out_total = 0;
@@ -961,31 +962,41 @@ size_t Context::shells_host(const int *quartets, int nq, double *out)
         const int *s = quartets + 4 * t;
         for (int a = 0; a < 4; a++)
             if (s[a] < 0 || (size_t)s[a] >= bas_.size() / 8) throw std::invalid_argument("shell index out of range");
-        if (bas_[8 * s[0] + 3] != 1 || bas_[8 * s[1] + 3] != 1 || bas_[8 * s[2] + 3] != 1 || bas_[8 * s[3] + 3] != 1)
-            throw std::invalid_argument("shell blocks support segmented shells only");
-        const int I = cum[s[0]], J = cum[s[1]], K = cum[s[2]], L = cum[s[3]];
-        auto pidx = [&](int x, int y) {
-            if (x < y) std::swap(x, y);
-            return pair_of_ij_[(size_t)x * (x + 1) / 2 + y];
-        };
-        Req r;
-        r.bra = pidx(I, J);
-        r.ket = pidx(K, L);
-        for (int a = 0; a < 4; a++) r.dims[a] = shells_[cum[s[a]]].nf;
-        r.off_out = out_total;
-        out_total += (long long)r.dims[0] * r.dims[1] * r.dims[2] * r.dims[3];
-        r.swap_bk = false;
-        r.off_dev = -1;
-        if (r.bra >= 0 && r.ket >= 0) {
-            if (pairs_[r.bra].cls < pairs_[r.ket].cls) {
-                std::swap(r.bra, r.ket);
-                r.swap_bk = true;
-            }
-            r.off_dev = dev_total;
-            dev_total += (long long)r.dims[0] * r.dims[1] * r.dims[2] * r.dims[3];
-            by_cls[{pairs_[r.bra].cls, pairs_[r.ket].cls}].push_back(ListTask{r.bra, r.ket, r.off_dev});
+        int nctr[4], dims[4];
+        for (int a = 0; a < 4; a++) {
+            nctr[a] = bas_[8 * s[a] + 3];
+            dims[a] = shells_[cum[s[a]]].nf * nctr[a];
         }
-        reqs.push_back(r);
+        for (int c3 = 0; c3 < nctr[3]; c3++)
+            for (int c2 = 0; c2 < nctr[2]; c2++)
+                for (int c1 = 0; c1 < nctr[1]; c1++)
+                    for (int c0 = 0; c0 < nctr[0]; c0++) {
+                        Req r;
+                        const int cc[4] = {c0, c1, c2, c3};
+                        for (int a = 0; a < 4; a++) {
+                            r.ctr[a] = cc[a];
+                            r.vs[a] = cum[s[a]] + cc[a];
+                            r.nf[a] = shells_[r.vs[a]].nf;
+                            r.dims[a] = dims[a];
+                        }
+                        r.bra = pidx(r.vs[0], r.vs[1]);
+                        r.ket = pidx(r.vs[2], r.vs[3]);
+                        r.off_out = out_total;
+                        r.swap_bk = false;
+                        r.off_dev = -1;
+                        if (r.bra >= 0 && r.ket >= 0) {
+                            if (pairs_[r.bra].cls < pairs_[r.ket].cls) {
+                                std::swap(r.bra, r.ket);
+                                r.swap_bk = true;
+                            }
+                            r.off_dev = dev_total;
+                            dev_total += (long long)r.nf[0] * r.nf[1] * r.nf[2] * r.nf[3];
+                            by_cls[{pairs_[r.bra].cls, pairs_[r.ket].cls}].push_back(
+                                ListTask{r.bra, r.ket, r.off_dev});
+                        }
+                        reqs.push_back(r);
+                    }
+        out_total += (long long)dims[0] * dims[1] * dims[2] * dims[3];
     }
     std::vector<double> blocks(std::max<long long>(dev_total, 1));
     if (dev_total) {
@@ -1006,45 +1017,46 @@ size_t Context::shells_host(const int *quartets, int nq, double *out)
         for (auto p : bufs) dev_free(p);
         dev_free(dout);
     }
-    for (int t = 0; t < nq; t++) {
-        const Req &r = reqs[t];
-        const int *s = quartets + 4 * t;
-        const int d0 = r.dims[0], d1 = r.dims[1], d2 = r.dims[2], d3 = r.dims[3];
+    for (const Req &r : reqs) {
+        const int d0 = r.dims[0], d1 = r.dims[1], d2 = r.dims[2];
         double *o = out + r.off_out;
-        if (r.off_dev < 0) {
-            std::memset(o, 0, sizeof(double) * d0 * d1 * d2 * d3);
+        const int n0 = r.nf[0], n1 = r.nf[1], n2 = r.nf[2], n3 = r.nf[3];
+        const int o0 = n0 * r.ctr[0], o1 = n1 * r.ctr[1], o2 = n2 * r.ctr[2], o3 = n3 * r.ctr[3];
+        if (r.off_dev < 0) {  // no surviving primitive pair: the sub-block is zero
+            for (int l = 0; l < n3; l++)
+                for (int k = 0; k < n2; k++)
+                    for (int j = 0; j < n1; j++)
+                        for (int i = 0; i < n0; i++) o[(o0 + i) + d0 * ((o1 + j) + d1 * ((o2 + k) + d2 * (o3 + l)))] = 0.0;
             continue;
         }
         // stored block: pair X = (sa, sb), pair Y = (sc, sd), a fastest
         const PairH &X = pairs_[r.bra], &Y = pairs_[r.ket];
         const int na = shells_[X.sa].nf, nbf = shells_[X.sb].nf, nc = shells_[Y.sa].nf;
-        const int I = cum[s[0]], J = cum[s[1]], K = cum[s[2]], L = cum[s[3]];
-        for (int l = 0; l < d3; l++)
-            for (int k = 0; k < d2; k++)
-                for (int j = 0; j < d1; j++)
-                    for (int i = 0; i < d0; i++) {
-                        // component of each stored slot
+        const int b0 = r.swap_bk ? 2 : 0, k0 = r.swap_bk ? 0 : 2;
+        for (int l = 0; l < n3; l++)
+            for (int k = 0; k < n2; k++)
+                for (int j = 0; j < n1; j++)
+                    for (int i = 0; i < n0; i++) {
+                        // component of each stored slot: the stored bra pair
+                        // (sa, sb) comes from requested slots b0, b0 + 1
+                        // (in either order), the ket pair from k0, k0 + 1
                         int fi[4];
-                        const int sh[4] = {I, J, K, L};
                         const int fv[4] = {i, j, k, l};
-                        // bra of stored block = requested bra unless swapped
-                        const int b0 = r.swap_bk ? 2 : 0, k0 = r.swap_bk ? 0 : 2;
-                        // stored bra pair (sa, sb) from requested slots b0, b0+1
-                        if (sh[b0] == X.sa && (sh[b0 + 1] == X.sb)) {
+                        if (r.vs[b0] == X.sa && r.vs[b0 + 1] == X.sb) {
                             fi[0] = fv[b0];
                             fi[1] = fv[b0 + 1];
                         } else {
                             fi[0] = fv[b0 + 1];
                             fi[1] = fv[b0];
                         }
-                        if (sh[k0] == Y.sa && (sh[k0 + 1] == Y.sb)) {
+                        if (r.vs[k0] == Y.sa && r.vs[k0 + 1] == Y.sb) {
                             fi[2] = fv[k0];
                             fi[3] = fv[k0 + 1];
                         } else {
                             fi[2] = fv[k0 + 1];
                             fi[3] = fv[k0];
                         }
-                        o[i + d0 * (j + d1 * (k + d2 * l))] =
+                        o[(o0 + i) + d0 * ((o1 + j) + d1 * ((o2 + k) + d2 * (o3 + l)))] =
                             blocks[r.off_dev + fi[0] + na * (fi[1] + nbf * (fi[2] + nc * fi[3]))];
                     }
     }
@@ -1151,6 +1163,7 @@ void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
     int k = 0;
     auto stream_of = [&](int j) { return (j % nst) == 0 ? (hipStream_t)0 : side_[(j % nst) - 1]; };
     for (const ClassLaunch &L : launches_)
+        if (!L.twin)
         check(launch_grad(L.la, L.lb, L.lc, L.ld, 0, d_tasks_ + L.first, L.count, B, d_pp_idx_, d_shell_slot_,
                           d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, stream_of(k++)),
               "grad launch");

@@ -46,6 +46,12 @@ struct ClassLaunch {
     long long jbase = -1;  // Coulomb slab offset (-1: ket Coulomb block by global atomics)
     int ifirst = 0, icount = 0;  // slab reduction items of this launch
     int drow = 0;                // bra density rows staged in LDS in the last build
+    // task flags this launch evaluates (bit f: flag f; flag 0 FP64, 1 FP32,
+    // 2 skipped).  In mixed precision a class with FP32 kernels has two
+    // launches over the same tasks: the FP64 one (accept 1) and its twin with
+    // FP32 recurrences (accept 2); otherwise a launch takes flags 0 and 1.
+    int accept = 3;
+    int twin = 0;  // 1: the FP32 twin (its counts belong to the FP64 launch)
 };
 
 class Context {
@@ -68,12 +74,12 @@ public:
     // nshards = 1 (default) is the whole build.  Rebuilds the task lists.
     void set_shard(int nshards, int shard);
     int nshards() const { return nshards_; }
-    // Precision mode 0: FP64 everywhere.  Mode 1 (mixed): quartets with
-    // Q_ab Q_cd dmax < tau_mix use FP32 recurrences, the split being made
-    // once when the task lists are built (dmax <= 0: max|D| of the next
-    // density).  tau_skip > 0 drops, per build, the tasks whose
-    // density-weighted bound Q_ab Q_cd max(|D| over the six blocks) is below it.
-    void set_precision(int mode, double tau_mix, double tau_skip, double dmax = 0.0);
+    // Precision mode 0: FP64 everywhere.  Mode 1 (mixed): every build bounds
+    // each 64-ket task by the largest Q_ab Q_cd max(|D| over the six blocks
+    // the quartet reads) of its kets, from that build's density; tasks below
+    // tau_mix run FP32 recurrences.  tau_skip > 0 drops, per build, the tasks
+    // below it (density screening, both modes).
+    void set_precision(int mode, double tau_mix, double tau_skip);
     int precision() const { return precision_; }
     // Quartets of the last build by task flag (0 FP64, 1 FP32, 2 skipped).
     void flag_counts(long long *quartets3, long long *prim3);
@@ -152,8 +158,7 @@ private:
     unsigned char *d_gmask_ = nullptr;
     double *d_dsh_ = nullptr;            // shell-block max |D|, per build
     int precision_ = 0;
-    double tau_mix_ = 0.0, tau_skip_ = 0.0, dmax_ = 0.0;
-    double *d_dmax_ = nullptr;
+    double tau_mix_ = 0.0, tau_skip_ = 0.0;
     bool classified_ = false;            // flags valid for the last build
     std::vector<JKTask> h_tasks_;        // host copy (counts by flag)
     int ngroups_ = 0;

@@ -133,7 +133,6 @@ hipError_t launch_classify(const JKTask *tasks, int ntask, const int *task_group
                            hipStream_t st);
 hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, int ntask, const DevBasis &B,
                        double *out, int mode, int nao, hipStream_t st);
-hipError_t launch_absmax(const double *D, size_t n, double *out, hipStream_t st);
 hipError_t launch_symmetrize(const double *M, double *G, int nao, hipStream_t st);
 
 } // namespace librint

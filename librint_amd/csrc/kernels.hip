@@ -911,25 +911,6 @@ hipError_t launch_classify(const JKTask *tasks, int ntask, const int *task_group
     return hipGetLastError();
 }
 
-// max |D| over n doubles (non-negative doubles order like their bit patterns)
-__global__ void absmax_kernel(const double *__restrict__ D, size_t n, unsigned long long *__restrict__ out)
-{
-    double m = 0.0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        m = fmax(m, fabs(D[i]));
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
-}
-
-hipError_t launch_absmax(const double *D, size_t n, double *out, hipStream_t st)
-{
-    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, st, (unsigned int *)out, 2);
-    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
-    hipLaunchKernelGGL(absmax_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, st, D, n, (unsigned long long *)out);
-    return hipGetLastError();
-}
-
 __global__ void symmetrize_kernel(const double *__restrict__ M, double *__restrict__ G, int nao)
 {
     const size_t n = (size_t)nao;

@@ -479,6 +479,33 @@ def evaluate(net, mols, log=None):
             "converged_all": bool(np.all(np.isfinite([r[1] for r in rows] + [r[3] for r in rows])))}
 
 
+def run(n_train, n_test, epochs=40, seed=2024, log=None):
+    """The config-3 experiment: train on n_train perturbed C2-C6 alkanes
+    (converged GPU-SCF densities), then compare SCF iteration counts from the
+    network's guess and from the core-Hamiltonian guess on n_test held-out
+    molecules drawn from the seed + 1 stream."""
+    t0 = time.time()
+    if log:
+        log("training set: %d molecules" % n_train)
+    X, T, W, _ = make_dataset(n_train, seed, log=log)
+    t_data = time.time() - t0
+    if log:
+        log("pairs %d; training" % X.shape[0])
+    t1 = time.time()
+    net, hist = train(X, T, W, epochs=epochs, log=log)
+    t_train = time.time() - t1
+    if log:
+        log("held-out set: %d molecules" % n_test)
+    rng = np.random.default_rng(seed + 1)
+    tests = [alkane_sample(int(rng.choice((2, 3, 4, 5, 6))), rng) for _ in range(n_test)]
+    res = evaluate(net, tests, log=log)
+    res.update({"train_molecules": n_train, "train_pairs": int(X.shape[0]), "epochs": epochs,
+                "first_train_mse": hist[0], "final_train_mse": hist[-1], "data_seconds": t_data,
+                "train_seconds": t_train,
+                "scf_convergence": "|dP|_F/2 <= 1e-5 and |dE| <= 1e-8 Eh, DIIS, 6-31G"})
+    return res
+
+
 def main(argv=None):
     import argparse
     import json
@@ -493,21 +520,7 @@ def main(argv=None):
     def log(s):
         print(s, file=sys.stderr, flush=True)
 
-    t0 = time.time()
-    log("training set: %d molecules" % a.train)
-    X, T, W, _ = make_dataset(a.train, a.seed, log=log)
-    t_data = time.time() - t0
-    log("pairs %d; training" % X.shape[0])
-    t1 = time.time()
-    net, hist = train(X, T, W, epochs=a.epochs, log=log)
-    t_train = time.time() - t1
-    log("held-out set: %d molecules" % a.test)
-    rng = np.random.default_rng(a.seed + 1)
-    tests = [alkane_sample(int(rng.choice((2, 3, 4, 5, 6))), rng) for _ in range(a.test)]
-    res = evaluate(net, tests, log=log)
-    res.update({"train_molecules": a.train, "train_pairs": int(X.shape[0]), "epochs": a.epochs,
-                "final_train_mse": hist[-1], "data_seconds": t_data, "train_seconds": t_train,
-                "scf_convergence": "|dP|_F/2 <= 1e-5 and |dE| <= 1e-8 Eh, DIIS, 6-31G"})
+    res = run(a.train, a.test, a.epochs, a.seed, log=log)
     print(json.dumps(res), flush=True)
 
 

@@ -128,6 +128,19 @@ def fock2e_sample(m, P, quartets):
     return G
 
 
+def fock2e_sample_threaded(m, P, quartets, nthreads=None):
+    """fock2e_sample over the quartet list split into chunks, one host
+    thread each (the C call releases the interpreter lock); partial G's
+    summed."""
+    from concurrent.futures import ThreadPoolExecutor
+    q = np.ascontiguousarray(quartets, dtype=np.int32).reshape(-1, 4)
+    nt = max(1, min(nthreads or host_threads(), len(q)))
+    chunks = np.array_split(q, nt)
+    with ThreadPoolExecutor(max_workers=nt) as ex:
+        parts = list(ex.map(lambda c: fock2e_sample(m, P, c), chunks))
+    return np.sum(parts, axis=0)
+
+
 def host_threads():
     """Threads for the threaded comparator: the cores this process may use,
     at most OMP_NUM_THREADS when that is set."""

@@ -143,19 +143,17 @@ def test_adam_and_conversion(gpu):
 
 
 def test_trained_guess_cuts_iterations_same_energy(gpu):
-    """A network trained on 160 perturbed C2-C4 alkanes (densities from the
-    GPU SCF) against the core-Hamiltonian guess on 12 held-out molecules:
-    fewer SCF iterations, the same converged energies."""
+    """The learned-guess bar at reduced size: a network trained (on the GPU,
+    by the library's own BF16 MFMA kernels) on 512 seeded perturbed C2-C6
+    alkanes, whose converged densities come from the GPU SCF, against the
+    core-Hamiltonian guess on 64 held-out C2-C6 alkanes.  It must cut the SCF
+    iterations by at least 30 % and leave every converged energy within
+    1e-10 Eh of the core-Hamiltonian-guess one."""
     from librint_amd import guess as G
-    X, T, W, _ = G.make_dataset(160, seed=11, carbons=(2, 3, 4))
-    _say("pairs", X.shape[0])
-    net, hist = G.train(X, T, W, epochs=30, batch=4096, lr=2e-3)
-    _say("mse first/last", hist[0], hist[-1])
-    assert hist[-1] < 0.2 * hist[0]
-    rng = np.random.default_rng(99)
-    tests = [G.alkane_sample(int(rng.choice((2, 3, 4))), rng) for _ in range(12)]
-    res = G.evaluate(net, tests)
+    res = G.run(512, 64, epochs=20, seed=2024, log=_say)
     _say(res)
+    assert res["final_train_mse"] < 0.2 * res["first_train_mse"]
     assert res["converged_all"]
-    assert res["max_energy_difference_Eh"] < 1e-7
-    assert res["iterations_mlp"] < res["iterations_core_h"]
+    assert res["molecules"] == 64
+    assert res["max_energy_difference_Eh"] <= 1e-10
+    assert res["reduction"] >= 0.30
