@@ -59,9 +59,10 @@ double scf_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t
  * respect to env[s1..s2) (s1, s2 from the split rule of reference
  * src/utils.rs:152-175: every primitive exponent and contraction
  * coefficient), P the density they are evaluated at; release with free_c.
- * The two-electron part runs on the GPU (adjoint Rys kernel), the
- * one-electron parts on the host; shells with l <= 2 (NULL and an error
- * message for l > 2).
+ * Both the two-electron part (adjoint Rys kernels) and the one-electron
+ * derivatives (S, T, V) run on the GPU; shells with l <= 4, the reference's
+ * domain (reference python/librint/dscf.py:11; NULL and an error message
+ * above it).
  *   dS_u          nao*nao*(s2-s1), [(a*nao+b)*L + k] = dS_ab/denv[s1+k]
  *                 reference src/p2c.rs:130-144, src/dscf.rs:96-150
  *   grad_c        dHcore + dR            reference src/p2c.rs:240-258, src/dscf.rs:547-557
@@ -145,16 +146,13 @@ int librint_schwarz(librint_ctx *ctx, double *out);
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 
-/* Per angular-class launch of the current plan, 14 values each:
+/* Per angular-class launch of the current plan, 13 values each:
  * la, lb, lc, ld, quartets, primitive quartets, algorithmic flops, lanes
  * launched (64 x tasks), device ms of the launch in the last host-side build
- * (librint_fock), task count, FP32 recurrences (0/1), kernel of the last
- * build (0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel for
- * classes with an f or g shell, -1 none yet),
- * bra density rows staged in LDS (0/1), task flags accepted (bit 0: FP64
- * tasks, bit 1: FP32 tasks; in mixed precision a class with FP32 kernels has
- * an FP64 launch (1) and an FP32 twin (2) over the same tasks, the counts
- * being those of the whole class in both).
+ * (librint_fock), task count, mixed (1: the launch picks FP32 recurrences per
+ * task in mixed precision), kernel of the last build (0 jk_kernel,
+ * 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel for classes with an f or
+ * g shell, -1 none yet), bra density rows staged in LDS (0/1).
  * Returns the number of launches (out may be NULL to query it); writes at
  * most n launches. */
 int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);
@@ -170,7 +168,7 @@ int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);
  *                  contraction coefficient (dHcore + dR - dS/2)
  * parts (optional): [gH | gR | gS | gN] (3*natm each) then [dH | dR | dS]
  * (env_l each).  Quartets are screened with the context's Schwarz threshold.
- * Shells with l <= 2 (returns -1 otherwise).
+ * Shells with l <= 4 (returns -1 otherwise).
  * No counterpart in the reference (its gradients are basis-parameter only). */
 int librint_ctx_gradient(librint_ctx *ctx, const double *P, double *gnuc, double *denv, double *parts);
 

@@ -101,21 +101,21 @@ class FockContext:
     def class_stats(self):
         """Per angular-class launch: counts, flops and the last build's device ms."""
         n = library.librint_ctx_class_stats(self.h, None, 0)
-        out = np.zeros(14 * max(n, 1))
+        out = np.zeros(13 * max(n, 1))
         library.librint_ctx_class_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
-        keys = ("la", "lb", "lc", "ld", "quartets", "prim_quartets", "flops", "lanes", "ms", "tasks", "fp32",
-                "variant", "drow", "accept")
-        rows = [dict(zip(keys, out[14 * k:14 * k + 14].tolist())) for k in range(n)]
+        keys = ("la", "lb", "lc", "ld", "quartets", "prim_quartets", "flops", "lanes", "ms", "tasks", "mixed",
+                "variant", "drow")
+        rows = [dict(zip(keys, out[13 * k:13 * k + 13].tolist())) for k in range(n)]
         for r in rows:
             v = int(r["variant"])
             ls = "%d, %d, %d, %d" % (r["la"], r["lb"], r["lc"], r["ld"])
-            t = "float" if r["fp32"] else "double"
             if v == 2:
                 r["kernel"] = "jk_team_kernel<%s>" % ls
             elif v == 1:
-                r["kernel"] = "jk_group_kernel<%s, %s, %s>" % (ls, t, "true" if r["drow"] else "false")
+                r["kernel"] = "jk_group_kernel<%s, %s, %s>" % (ls, "true" if r["mixed"] else "false",
+                                                              "true" if r["drow"] else "false")
             elif v == 0:
-                r["kernel"] = "jk_kernel<%s, %s>" % (ls, t)
+                r["kernel"] = "jk_kernel<%s, double>" % ls
             elif v == 3:
                 r["kernel"] = "hl_kernel<3>"  # run-time class shape (an f or g shell)
             else:

@@ -136,6 +136,7 @@ Context::~Context()
     dev_free(d_gmask_w_);
     dev_free(d_gmask_);
     dev_free(d_dsh_);
+    dev_free(d_mixb_);
     dev_free(d_counters_);
     dev_free(d_M_);
     dev_free(d_D_);
@@ -375,9 +376,13 @@ void Context::set_shard(int nshards, int shard)
 // Per primitive quartet flop count of the fp64 kernel's algorithm (the
 // definition used for roofline numbers): root evaluation, recurrence
 // coefficients, VRR, HRR and the gout products, FMA counted as 2 flops.
+// An all-s quartet evaluates the weight polynomial only (degree 11, Estrin:
+// 11 FMA) after the segment variable (6), no node and no recurrence
+// coefficient, and adds w fac1 (1 mul) into its one integral (1 add).
 static double flops_prim(int la, int lb, int lc, int ld)
 {
     const int nmax = la + lb, mmax = lc + ld, nr = (nmax + mmax) / 2 + 1;
+    if (nmax + mmax == 0) return 30 + 6 + 22 + 1 + 1;
     const int dk = la + 1, dl = dk * (lc + 1);
     (void)dl;
     const int nf = ncart(la) * ncart(lb) * ncart(lc) * ncart(ld);
@@ -411,7 +416,9 @@ void Context::build_tasks()
     // A group = consecutive tasks of one bra pair for one ket class; its size
     // is set per launch below (LIBRINT_GROUP_TASKS overrides).
     int GROUP_MAX_TASKS = 0;
-    if (const char *e = getenv("LIBRINT_GROUP_TASKS")) GROUP_MAX_TASKS = std::max(1, atoi(e));
+    // (at most 256: a wavefront of the group kernel holds the flags of its
+    // quarter of a group's tasks, one per lane)
+    if (const char *e = getenv("LIBRINT_GROUP_TASKS")) GROUP_MAX_TASKS = std::max(1, std::min(256, atoi(e)));
     struct Acc {
         std::vector<JKTask> tasks;   // contiguous per bra pair
         std::vector<double> cost;    // primitive quartets per task
@@ -419,9 +426,8 @@ void Context::build_tasks()
         ClassLaunch stat;
     };
     std::map<std::tuple<int, int, int>, Acc> acc;  // (bra class, ket class, 0)
-    // mixed precision: the classes with FP32 kernels get a second launch over
-    // the same tasks (see below); which of the two evaluates a task is decided
-    // per build from that build's density
+    // mixed precision: the classes with FP32 kernels launch the form that
+    // picks FP64 or FP32 recurrences per task from each build's density
     const bool mixed = precision_ == 1 && tau_mix_ > 0.0;
     for (int X = 0; X < nb; X++) {
         const Bin &bx = bins_[X];
@@ -570,7 +576,7 @@ void Context::build_tasks()
         L.lb = lb;
         L.lc = lc;
         L.ld = ld;
-        L.fp32 = std::get<2>(kv.first);
+        L.fp32 = mixed && class_has_fp32(la, lb, lc, ld) ? 1 : 0;
         L.first = (int)tasks.size();
         L.gfirst = (int)groups.size();
         for (auto &g : A.groups) {
@@ -585,7 +591,6 @@ void Context::build_tasks()
         }
         L.count = (int)tasks.size() - L.first;
         L.gcount = (int)groups.size() - L.gfirst;
-        L.accept = mixed && class_has_fp32(la, lb, lc, ld) ? 1 : 3;
         if (L.count) launches_.push_back(L);
     }
     stats_.launches = (int)launches_.size();
@@ -645,16 +650,6 @@ void Context::build_tasks()
         L.icount = (int)items.size() - L.ifirst;
     }
     nitems_ = (int)items.size();
-    // the FP32 twins: same tasks, groups, slabs and reduction items
-    for (size_t k = 0, nl = launches_.size(); k < nl; k++)
-        if (launches_[k].accept == 1) {
-            ClassLaunch T = launches_[k];
-            T.fp32 = 1;
-            T.accept = 2;
-            T.twin = 1;
-            launches_.push_back(T);
-        }
-    stats_.launches = (int)launches_.size();
     dev_free(d_items_);
     dev_free(d_item_tasks_);
     d_items_ = dev_upload(items);
@@ -705,8 +700,8 @@ void Context::set_precision(int mode, double tau_mix, double tau_skip)
 {
     if (mode != 0 && mode != 1) throw std::invalid_argument("precision mode must be 0 (FP64) or 1 (mixed)");
     if (!(tau_mix >= 0) || !(tau_skip >= 0)) throw std::invalid_argument("thresholds must be >= 0");
-    // the lists change only when the FP32 twins appear or go; a new
-    // threshold is applied by the next build's classification
+    // the lists change only when the mixed kernels come or go; a new
+    // threshold is applied by the next build
     const bool rebuild = (mode == 1 && tau_mix > 0.0) != (precision_ == 1 && tau_mix_ > 0.0);
     precision_ = mode;
     tau_mix_ = tau_mix;
@@ -718,14 +713,17 @@ void Context::flag_counts(long long *q3, long long *p3)
 {
     for (int f = 0; f < 3; f++) q3[f] = p3[f] = 0;
     std::vector<unsigned char> fl(h_tasks_.size(), 0);
-    if (classified_ && !fl.empty())
+    if ((classified_ || mixed_last_) && !fl.empty())
         check(hipMemcpy(fl.data(), d_flags_, fl.size(), hipMemcpyDeviceToHost), "hipMemcpy D2H");
     for (const ClassLaunch &L : launches_) {
-        if (L.twin) continue;  // counted with its FP64 launch
         for (int t = L.first; t < L.first + L.count; t++) {
             const JKTask &T = h_tasks_[t];
-            // flag 1 is evaluated in FP32 only where the class has an FP32 twin
-            const int f = fl[t] == 2 ? 2 : (L.accept == 1 ? fl[t] : 0);
+            // a mixed launch records 0 (FP64) or 1 (FP32) per evaluated task;
+            // the other launches' tasks hold a classification flag (0 or 2)
+            // or, without classification, nothing
+            int f = 0;
+            if (L.fp32 && L.variant == 1 && mixed_last_) f = fl[t];
+            else if (classified_) f = fl[t] == 2 ? 2 : 0;
             int b0, b1;
             task_prims(T, pairs_[T.bra].npp, b0, b1);
             if (b0 == 0) q3[f] += T.nket;  // a split quartet counts once
@@ -734,31 +732,107 @@ void Context::flag_counts(long long *q3, long long *p3)
     }
 }
 
-void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
+void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st) { fock_impl(D_dev, G_dev, st, nullptr); }
+
+size_t Context::scratch_doubles_needed() const
+{
+    const size_t nsh = shells_.size(), np = pairs_.size();
+    const bool mixed = precision_ == 1 && tau_mix_ > 0.0;
+    const bool dsh = tau_skip_ > 0.0 || mixed;
+    // M, Dsh, the mixed-precision bounds, the counters
+    size_t need = (size_t)nao_ * nao_ + (dsh ? nsh * nsh : 0) + (mixed ? nsh + 3 * np : 0) + launches_.size();
+    bool hl = false;
+    size_t hl_nf = 0;
+    for (const ClassLaunch &L : launches_)
+        if (hl_class(L.la, L.lb, L.lc, L.ld)) {
+            hl = true;
+            hl_nf = std::max<size_t>(hl_nf, (size_t)hl_shape(L.la, L.lb, L.lc, L.ld).nf);
+        }
+    return need + (hl ? hl_nf : 0);
+}
+
+void Context::fock_device_scratch(const double *D_dev, double *G_dev, hipStream_t st, double *scratch, size_t len)
+{
+    const size_t need = scratch_doubles_needed();
+    if (need > len)
+        throw std::invalid_argument("the Fock build needs " + std::to_string(need) +
+                                    " doubles of scratch, more than the " + std::to_string(len) + " available");
+    FockScratch sc;
+    const size_t nn = (size_t)nao_ * nao_, nsh = shells_.size(), np = pairs_.size();
+    const bool mixed = precision_ == 1 && tau_mix_ > 0.0;
+    const bool dsh = tau_skip_ > 0.0 || mixed;
+    double *p = scratch;
+    sc.M = p;
+    p += nn;
+    sc.dsh = dsh ? p : nullptr;
+    p += dsh ? nsh * nsh : 0;
+    sc.mix = mixed ? p : nullptr;
+    p += mixed ? nsh + 3 * np : 0;
+    sc.counters = (int *)p;
+    p += launches_.size();
+    sc.hl = p;
+    sc.hl_len = len - (size_t)(p - scratch);
+    fock_impl(D_dev, G_dev, st, &sc);
+}
+
+void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, const FockScratch *sc)
 {
     const size_t nn = (size_t)nao_ * nao_;
-    if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
-    check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
+    double *M = nullptr;
+    int *counters = d_counters_;
+    if (sc) {
+        // caller-provided scratch: nothing is allocated; M is cleared with a
+        // kernel (no memset node if the caller captures the stream)
+        M = sc->M;
+        counters = sc->counters;
+        check(launch_zero_doubles(M, nn, st), "zero M");
+    } else {
+        if (!d_M_) check(hipMalloc(&d_M_, sizeof(double) * nn), "hipMalloc");
+        M = d_M_;
+        check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
+    }
     const DevBasis B = devb();
-    check(launch_zero_counters(d_counters_, 2 * (int)launches_.size(), st), "zero counters");
+    check(launch_zero_counters(counters, 2 * (int)launches_.size(), st), "zero counters");
     if (max_lds_ == 0) {
         int dev = 0;
         check(hipGetDevice(&dev), "hipGetDevice");
         check(hipDeviceGetAttribute(&max_lds_, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
         if (getenv("LIBRINT_DEBUG")) fprintf(stderr, "librint: max LDS per block %d\n", max_lds_);
     }
-    // Per-build task flags from the density: skipped below tau_skip, FP32
-    // below tau_mix where the class has an FP32 twin launch
+    // Per-build task flags from the density's shell-block maxima: each task's
+    // largest bound Q_ab Q_cd max(|D| over the quartet's six blocks) below
+    // tau_skip: 2 (skipped); below tau_mix in mixed precision: 1 (the mixed
+    // kernels run it with FP32 recurrences, the others in FP64); else 0
     const bool mixed = precision_ == 1 && tau_mix_ > 0.0;
-    const bool classify = tau_skip_ > 0.0 || mixed;
+    const bool classify = tau_skip_ > 0.0;
     const int nsh = (int)shells_.size();
-    if (classify) {
-        if (!d_dsh_) check(hipMalloc(&d_dsh_, sizeof(double) * (size_t)nsh * nsh), "hipMalloc");
-        check(launch_classify(d_tasks_, (int)h_tasks_.size(), d_task_group_, ngroups_, B, D_dev, d_ao_off_, nsh, nao_,
-                              d_dsh_, mixed ? tau_mix_ : 0.0, tau_skip_, d_flags_, d_gmask_w_, d_gmask_, st),
-              "classify launch");
+    const int np = (int)pairs_.size();
+    MixArgs mx{nullptr, nullptr, tau_mix_, d_flags_};
+    if (classify || mixed) {
+        double *dsh = sc ? sc->dsh : d_dsh_;
+        if (!sc && !d_dsh_) {
+            check(hipMalloc(&d_dsh_, sizeof(double) * (size_t)nsh * nsh), "hipMalloc");
+            dsh = d_dsh_;
+        }
+        check(launch_dsh(D_dev, d_ao_off_, nsh, nao_, dsh, st), "dsh launch");
+        if (classify)
+            check(launch_classify(d_tasks_, (int)h_tasks_.size(), d_task_group_, ngroups_, B, D_dev, d_ao_off_, nsh,
+                                  nao_, dsh, 0.0, tau_skip_, d_flags_, d_gmask_w_, d_gmask_, st),
+                  "classify launch");
+        if (mixed) {
+            double *mb = sc ? sc->mix : d_mixb_;
+            if (!sc && !d_mixb_) {
+                check(hipMalloc(&d_mixb_, sizeof(double) * ((size_t)nsh + 3 * (size_t)np)), "hipMalloc");
+                mb = d_mixb_;
+            }
+            double *drow = mb, *dpair = mb + nsh, *qd = dpair + np, *wqd = qd + np;
+            check(launch_mix_bounds(dsh, nsh, B, np, drow, dpair, qd, wqd, st), "mixed-precision bounds launch");
+            mx.Dpair = dpair;
+            mx.Wqd = wqd;
+        }
     }
     classified_ = classify;
+    mixed_last_ = mixed;
     const unsigned char *flags = classify ? d_flags_ : nullptr;
     const unsigned char *gmask = classify ? d_gmask_ : nullptr;
     check(hipEventRecord(ev0_, st), "hipEventRecord");
@@ -768,17 +842,20 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
     // but M (atomics) and, in the per-launch slab mode, their stream's slab
     // buffer.
     const int nl = (int)launches_.size();
-    // measured: two streams are best for the FP64 lists, four for the mixed
-    // ones (twice the launches, many of them small)
-    const int nstreams = nstreams_ > 0 ? nstreams_ : (precision_ == 1 ? 4 : 2);
-    {
+    // measured: two streams are best (three and four were slower)
+    const int nstreams = nstreams_ > 0 ? nstreams_ : 2;
+    if (!sc) {
         std::vector<std::array<int, 4>> cl;
         for (const ClassLaunch &L : launches_) cl.push_back({L.la, L.lb, L.lc, L.ld});
         hl_scratch_for(cl);
     }
+    // with caller scratch: one stream (nothing to create), no Coulomb slabs
+    // (the ket Coulomb block goes to M by atomics)
+    const bool slabs = sc == nullptr;
     int ns = std::min(nstreams, nl - 1);
     if (!jslab_whole_) ns = std::min(ns, slab_streams_);
     ns = std::max(ns, 1);
+    if (sc) ns = 1;
     if (ns > 1) ensure_side_streams(ns - 1);
     for (int j = 0; j < nl; j++) {
         const int k = j == 0 ? dominant_ : (j <= dominant_ ? j - 1 : j);
@@ -793,7 +870,7 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         const hipStream_t sk = slot == 0 ? st : side_[slot - 1];
         double *jbuf = jslab_whole_ ? d_J_ : d_J_ + (size_t)slot * jslot_;
         check(hipEventRecord(evs_[k], sk), "hipEventRecord");
-        int *ctr = d_counters_ + 2 * k;
+        int *ctr = counters + 2 * k;
         const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, false);
         const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
         // Staging the bra's density rows in LDS (DROW) doubles the row space
@@ -812,24 +889,24 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         L.drow = drow ? 1 : 0;
         const bool hl = hl_class(L.la, L.lb, L.lc, L.ld);
         L.variant = hl ? 3 : (grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2));
-        double *js = L.jbase >= 0 ? jbuf + L.jbase : nullptr;
+        double *js = (slabs && L.jbase >= 0) ? jbuf + L.jbase : nullptr;
         if (hl) {
             const HLShape S = hl_shape(L.la, L.lb, L.lc, L.ld);
-            check(launch_hl_jk(S, d_tasks_ + L.first, L.count, flags ? flags + L.first : nullptr, L.accept, B, D_dev, d_M_,
-                               nao_, hl_scratch(S), nhlscr_, sk),
+            check(launch_hl_jk(S, d_tasks_ + L.first, L.count, flags ? flags + L.first : nullptr, 3, B, D_dev, M, nao_,
+                               sc ? sc->hl : hl_scratch(S), sc ? sc->hl_len : nhlscr_, sk),
                   "jk launch (l > 2)");
         } else if (grouped)
             check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
-                                  gmask ? gmask + L.gfirst : nullptr, L.accept, B, D_dev, d_M_, nao_, js, L.first, drow,
-                                  sk),
+                                  gmask ? gmask + L.gfirst : nullptr, 3, B, D_dev, M, nao_, js, L.first, drow,
+                                  mx, sk),
                   "jk group launch");
         else
             check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + L.first, L.count, ctr,
-                            flags ? flags + L.first : nullptr, L.accept, B, D_dev, d_M_, nao_, js, 0, sk),
+                            flags ? flags + L.first : nullptr, 3, B, D_dev, M, nao_, js, 0, sk),
                   "jk launch");
-        if (!jslab_whole_ && L.icount)
-            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, L.accept, jbuf, B,
-                                 d_M_, nao_, sk),
+        if (slabs && !jslab_whole_ && L.icount)
+            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, 3, jbuf, B,
+                                 M, nao_, sk),
                   "coulomb reduction launch");
         check(hipEventRecord(evc_[k], sk), "hipEventRecord");
     }
@@ -837,11 +914,11 @@ void Context::fock_device(const double *D_dev, double *G_dev, hipStream_t st)
         check(hipEventRecord(evj_[i], side_[i]), "hipEventRecord");
         check(hipStreamWaitEvent(st, evj_[i], 0), "hipStreamWaitEvent");
     }
-    if (jslab_whole_ && nitems_)
-        check(launch_jreduce(d_items_, nitems_, max_nfcd_, d_item_tasks_, d_tasks_, flags, 3, d_J_, B, d_M_, nao_, st),
+    if (slabs && jslab_whole_ && nitems_)
+        check(launch_jreduce(d_items_, nitems_, max_nfcd_, d_item_tasks_, d_tasks_, flags, 3, d_J_, B, M, nao_, st),
               "coulomb reduction launch");
     check(hipEventRecord(ev1_, st), "hipEventRecord");
-    check(launch_symmetrize(d_M_, G_dev, nao_, st), "symmetrize launch");
+    check(launch_symmetrize(M, G_dev, nao_, st), "symmetrize launch");
 }
 
 // At least n extra streams (default priority, non-blocking) with one join
@@ -1114,8 +1191,6 @@ void Context::build_grad_tasks()
 
 void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
 {
-    for (const Shell &S : shells_)
-        if (S.l > 2) throw std::invalid_argument("analytic gradients cover l <= 2");
     if (nshards_ > 1) throw std::invalid_argument("analytic gradients need the whole build (nshards = 1)");
     // per-shell primitive slots: exponent and coefficient derivatives are
     // accumulated per (shell, primitive) and folded onto env entries here
@@ -1162,15 +1237,28 @@ void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
     }
     int k = 0;
     auto stream_of = [&](int j) { return (j % nst) == 0 ? (hipStream_t)0 : side_[(j % nst) - 1]; };
-    for (const ClassLaunch &L : launches_)
-        if (!L.twin)
-        check(launch_grad(L.la, L.lb, L.lc, L.ld, 0, d_tasks_ + L.first, L.count, B, d_pp_idx_, d_shell_slot_,
-                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, stream_of(k++)),
-              "grad launch");
-    for (const GradLaunch &G : glaunch1_)
-        check(launch_grad(G.la, G.lb, G.lc, G.ld, 1, d_gtasks1_ + G.first, G.count, B, d_pp_idx_, d_shell_slot_,
-                          d_slot_coef_, d_D_, nao_, dg, dg + 3 * na, stream_of(k++)),
-              "grad launch");
+    // the f/g classes (run-time shaped kernel) share the scratch of one
+    // weight block per workgroup: they all run on the null stream, sized
+    // for the largest class before any launch
+    {
+        std::vector<std::array<int, 4>> cl;
+        for (const ClassLaunch &L : launches_) cl.push_back({L.la, L.lb, L.lc, L.ld});
+        hl_scratch_for(cl);
+    }
+    auto grad_launch = [&](int la, int lb, int lc, int ld, int side, const JKTask *tl, int cnt) {
+        if (hl_class(la, lb, lc, ld)) {
+            const HLShape S = hl_shape(la, lb, lc, ld);
+            check(launch_hl_grad(la, lb, lc, ld, side, tl, cnt, B, d_pp_idx_, d_shell_slot_, d_slot_coef_, d_D_,
+                                 nao_, dg, dg + 3 * na, hl_scratch(S), nhlscr_, (hipStream_t)0),
+                  "grad launch (l > 2)");
+        } else {
+            check(launch_grad(la, lb, lc, ld, side, tl, cnt, B, d_pp_idx_, d_shell_slot_, d_slot_coef_, d_D_, nao_,
+                              dg, dg + 3 * na, stream_of(k++)),
+                  "grad launch");
+        }
+    };
+    for (const ClassLaunch &L : launches_) grad_launch(L.la, L.lb, L.lc, L.ld, 0, d_tasks_ + L.first, L.count);
+    for (const GradLaunch &G : glaunch1_) grad_launch(G.la, G.lb, G.lc, G.ld, 1, d_gtasks1_ + G.first, G.count);
     for (int i = 0; i < nst - 1; i++) {
         check(hipEventRecord(evj_[i], side_[i]), "hipEventRecord");
         check(hipStreamWaitEvent(0, evj_[i], 0), "hipStreamWaitEvent");

@@ -38,7 +38,7 @@ struct ClassLaunch {
     int la, lb, lc, ld;
     int first, count;    // range in the device task array
     int gfirst, gcount;  // range in the device group array
-    int fp32 = 0;        // FP32 recurrences for every task of this launch
+    int fp32 = 0;        // mixed: FP32 recurrences for the tasks below tau_mix, decided per build
     long long quartets = 0, prim_quartets = 0, lanes = 0;
     double flops = 0;
     float ms = 0.f;    // device time of this launch in the last build
@@ -46,12 +46,6 @@ struct ClassLaunch {
     long long jbase = -1;  // Coulomb slab offset (-1: ket Coulomb block by global atomics)
     int ifirst = 0, icount = 0;  // slab reduction items of this launch
     int drow = 0;                // bra density rows staged in LDS in the last build
-    // task flags this launch evaluates (bit f: flag f; flag 0 FP64, 1 FP32,
-    // 2 skipped).  In mixed precision a class with FP32 kernels has two
-    // launches over the same tasks: the FP64 one (accept 1) and its twin with
-    // FP32 recurrences (accept 2); otherwise a launch takes flags 0 and 1.
-    int accept = 3;
-    int twin = 0;  // 1: the FP32 twin (its counts belong to the FP64 launch)
 };
 
 class Context {
@@ -86,6 +80,13 @@ public:
 
     // G = J - K/2 for a symmetric density (device pointers, nao*nao, row-major).
     void fock_device(const double *D_dev, double *G_dev, hipStream_t st);
+    // The same build using `len` doubles of caller-provided device scratch
+    // (the accumulator M, the shell-block density maxima, the work counters
+    // and the f/g classes' block scratch) instead of the context's buffers:
+    // nothing is allocated, one stream, no Coulomb slabs.  Throws when the
+    // scratch is smaller than scratch_doubles_needed().
+    void fock_device_scratch(const double *D_dev, double *G_dev, hipStream_t st, double *scratch, size_t len);
+    size_t scratch_doubles_needed() const;
     // Host-pointer convenience wrapper.
     void fock_host(const double *D, double *G);
 
@@ -118,6 +119,12 @@ private:
     void build_tasks();
     void upload();
     void ensure_side_streams(int n);
+    struct FockScratch {
+        double *M, *dsh, *mix, *hl;
+        int *counters;
+        size_t hl_len;
+    };
+    void fock_impl(const double *D_dev, double *G_dev, hipStream_t st, const FockScratch *sc);
 
     std::vector<int> atm_, bas_;
     std::vector<double> env_;
@@ -157,9 +164,11 @@ private:
     unsigned int *d_gmask_w_ = nullptr;  // per group, per build
     unsigned char *d_gmask_ = nullptr;
     double *d_dsh_ = nullptr;            // shell-block max |D|, per build
+    double *d_mixb_ = nullptr;           // mixed-precision bounds per build: Drow, Dpair, Q Dpair, Wqd
     int precision_ = 0;
     double tau_mix_ = 0.0, tau_skip_ = 0.0;
     bool classified_ = false;            // flags valid for the last build
+    bool mixed_last_ = false;            // the last build ran the mixed kernels (flags hold 0/1)
     std::vector<JKTask> h_tasks_;        // host copy (counts by flag)
     int ngroups_ = 0;
     int *d_counters_ = nullptr;  // one work counter per class launch
@@ -203,7 +212,7 @@ private:
     // caller's stream and nstreams_ - 1 extra streams (LIBRINT_STREAMS),
     // joined before the Coulomb reduction: small launches and launch tails
     // overlap
-    int nstreams_ = 0;      // 0: 2 for FP64 lists, 4 for mixed ones
+    int nstreams_ = 0;      // 0: 2
     int team_kets_ = 8;     // ket window of the tasks of the team and f/g classes (LIBRINT_TEAM_KETS)
     // lane-per-quartet classes: a task whose lanes would each evaluate more
     // than prim_split_ primitive quartets is split by bra primitive pairs

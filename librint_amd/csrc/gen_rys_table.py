@@ -39,7 +39,9 @@ import multiprocessing as mpr
 import mpmath as mp
 
 F32 = "--f32" in sys.argv
-NMAX = 9          # roots up to an all-g quartet (l <= 4)
+# roots up to the adjoint of an all-g quartet (l <= 4 raised by 2 on one
+# pair: 10 roots); the FP32 table serves the l <= 2 recurrences only
+NMAX = 9 if F32 else 10
 DEG = 5 if F32 else 11  # polynomial degree per segment
 SEGLEN = 1.0      # segment length in x
 DPS = 50

@@ -40,4 +40,16 @@ hipError_t launch_hl_jk(const HLShape &S, const JKTask *tasks, int ntask, const 
 hipError_t launch_hl_list(const HLShape &S, const ListTask *tasks, int ntask, const DevBasis &B, double *out, int mode,
                           int nao, double *scr, size_t scr_len, hipStream_t st);
 
+// Shape of the adjoint's recurrence for the class (xa xb | ya yb) with the
+// pair (xa, xb) differentiated (momentum raised by 2 on both its centres).
+HLShape hl_grad_shape(int xa, int xb, int ya, int yb);
+// Adjoint of E2 = 1/2 tr(P G[P]) over JKTasks of a class with an f or g
+// shell (side 0: the task's bra pair is differentiated, side 1: its fixed
+// pair is the canonical ket of a ket-major list); outputs and per-primitive
+// slots as launch_grad (grad.h).  scr as launch_hl_jk (one weight block of
+// nf doubles per workgroup).
+hipError_t launch_hl_grad(int la, int lb, int lc, int ld, int side, const JKTask *tasks, int ntask, const DevBasis &B,
+                          const int2 *pp_idx, const int2 *shell_info, const double *coef, const double *D, int nao,
+                          double *gatom, double *dprim, double *scr, size_t scr_len, hipStream_t st);
+
 }  // namespace librint

@@ -36,6 +36,16 @@ static constexpr int HL_CHAIN_LDS = 48 * 1024 + 1024;
 // J/K rows: six blocks of at most 15 x 15 entries
 static constexpr int HL_JK_ROWS = 6 * 225;
 static constexpr int HL_MAX_GROUPS = 8192;
+// LDS budget of the adjoint's raised chains (bytes): one root of the
+// raised (gg|gg) chains (3 x 3465 doubles) fits
+static constexpr int HL_GRAD_CHAIN_LDS = 96 * 1024;
+
+__device__ __forceinline__ double wave_sum64(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
 
 HLShape hl_shape(int la, int lb, int lc, int ld)
 {
@@ -332,6 +342,194 @@ __global__ __launch_bounds__(64) void hl_kernel(HLShape S, const JKTask *__restr
             __syncthreads();
         }
     }
+}
+
+// ---------------------------------------------------------- adjoint --
+// Adjoint of E2 = 1/2 tr(P G[P]) for the classes with an f or g shell, the
+// run-time shaped form of grad_kernel (grad.hip; same energy weights
+// W_f = s (4 P_ab P_cd - P_ac P_bd - P_ad P_bc), same derivative identities,
+// same task lists: side 0 the Fock build's bra-major tasks, side 1 the
+// ket-major ones).  The differentiated pair X (the task's fixed pair) takes
+// the bra role of a recurrence raised by 2 on both its centres; rb roots of
+// its (root, axis) chains at a time are built in LDS by lanes c < 3 rb, then
+// every lane contracts its outputs f = lane + 64 k.  The weights of the
+// current quartet live in the workgroup's global scratch slice (up to 50625
+// of them).
+// Shape of the raised recurrence of class (xa xb | ya yb), X = (xa, xb) raised.
+static int hl_groups(const HLShape &S, int ntask, size_t scr_len);
+
+HLShape hl_grad_shape(int xa, int xb, int ya, int yb)
+{
+    HLShape S;
+    S.la = xa + 2;  // HRR b -> a runs to a power xa + 2
+    S.lb = xb + 2;  // the d -> c HRR keeps b powers up to xb + 2
+    S.lc = ya;
+    S.ld = yb;
+    S.nfa = ncart(xa);
+    S.nfb = ncart(xb);
+    S.nfc = ncart(ya);
+    S.nfd = ncart(yb);
+    S.nf = S.nfa * S.nfb * S.nfc * S.nfd;
+    S.nmax = xa + xb + 2;
+    S.mmax = ya + yb;
+    S.nroots = (xa + xb + ya + yb + 2) / 2 + 1;
+    S.dk = xa + 3;
+    S.dl = S.dk * (ya + 1);
+    S.dj = S.dl * (S.mmax + 1);
+    S.gs = S.dj * (S.nmax + 1);
+    const int per_root = 3 * S.gs * (int)sizeof(double);
+    S.rb = std::max(1, std::min(S.nroots, HL_GRAD_CHAIN_LDS / per_root));
+    return S;
+}
+
+__global__ __launch_bounds__(64) void hl_grad_kernel(HLShape R, int la, int lb, int lc, int ld, int side,
+                                                     const JKTask *__restrict__ tasks, int ntask, DevBasis B,
+                                                     const int2 *__restrict__ pp_idx,
+                                                     const int2 *__restrict__ shell_info,
+                                                     const double *__restrict__ coef, const double *__restrict__ D,
+                                                     int nao, double *__restrict__ gatom, double *__restrict__ dprim,
+                                                     double *__restrict__ scr_all)
+{
+    extern __shared__ double g[];
+    const int lane = threadIdx.x;
+    const int nfa = ncart(la), nfb = ncart(lb), nfc = ncart(lc), nfd = ncart(ld);
+    const int nf = nfa * nfb * nfc * nfd;
+    double *W = scr_all + (size_t)blockIdx.x * nf;
+    const size_t n = (size_t)nao;
+    for (int ti = blockIdx.x; ti < ntask; ti += gridDim.x) {
+        const JKTask t = tasks[ti];
+        const PairView sd = load_pair(B, t.bra);  // the differentiated pair
+        const int2 si0 = shell_info[sd.sa], si1 = shell_info[sd.sb];
+        int o_begin = 0, o_end = sd.npp;
+        if (side == 0) task_prims(t, sd.npp, o_begin, o_end);
+        double gx0[3] = {0.0, 0.0, 0.0}, gx1[3] = {0.0, 0.0, 0.0};
+        for (int kk = 0; kk < t.nket; kk++) {
+            const int otp = t.ket0 + kk;
+            const PairView ot = load_pair(B, otp);
+            const PairView &br = side == 0 ? sd : ot;  // canonical bra (a b|
+            const PairView &kt = side == 0 ? ot : sd;  // canonical ket |c d)
+            double s = 1.0;
+            if (br.sa == br.sb) s *= 0.5;
+            if (kt.sa == kt.sb) s *= 0.5;
+            if (t.bra == otp) s *= 0.5;
+            for (int f = lane; f < nf; f += 64) {
+                const int fa = f % nfa, t1 = f / nfa, fb = t1 % nfb, t2 = t1 / nfb, fc = t2 % nfc, fd = t2 / nfc;
+                const size_t a = br.aoa + fa, b = br.aob + fb, c = kt.aoa + fc, d = kt.aob + fd;
+                W[f] = s * (4.0 * D[a * n + b] * D[c * n + d] - D[a * n + c] * D[b * n + d] - D[a * n + d] * D[b * n + c]);
+            }
+            const double rxy[3] = {sd.rb[0] - ot.rb[0], sd.rb[1] - ot.rb[1], sd.rb[2] - ot.rb[2]};
+            for (int o = o_begin; o < o_end; o++) {
+                const PrimPair po = B.pp[sd.off + o];
+                const double al0 = po.pad0, al1 = po.pad1;  // exponents of the two centres of X
+                double e0 = 0.0, e1 = 0.0, val = 0.0;
+                for (int i = 0; i < ot.npp; i++) {
+                    const PrimPair pi = B.pp[ot.off + i];
+                    const double aij = po.e, akl = pi.e;
+                    const double rsq = rsqrt_pos(aij + akl);
+                    const double rs = rsq * rsq;
+                    const double a0 = aij * akl * rs;
+                    const double pq[3] = {po.pbx - pi.pbx + rxy[0], po.pby - pi.pby + rxy[1], po.pbz - pi.pbz + rxy[2]};
+                    const double rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
+                    const double fac1 = po.K * pi.K * rsq;
+                    const double x = a0 * rr;
+                    for (int r0 = 0; r0 < R.nroots; r0 += R.rb) {
+                        const int nb = min(R.rb, R.nroots - r0);
+                        if (lane < 3 * nb) {
+                            const int r = r0 + lane / 3, ax = lane - 3 * (lane / 3);
+                            double t2, wr;
+                            rys_one_rt(R.nroots, x, r, t2, wr);
+                            const double tr = t2 * rs;
+                            const double b00 = 0.5 * tr;
+                            const double b10 = po.h * (1.0 - akl * tr);
+                            const double b01 = pi.h * (1.0 - aij * tr);
+                            const double pqa = sel3(ax, pq[0], pq[1], pq[2]);
+                            const double cn = sel3(ax, po.pbx, po.pby, po.pbz) - akl * tr * pqa;
+                            const double cm = sel3(ax, pi.pbx, pi.pby, pi.pbz) + aij * tr * pqa;
+                            double *gc = g + lane * R.gs;
+                            gc[0] = (ax == 2) ? wr * fac1 : 1.0;
+                            vrr_rt(gc, R, cn, cm, b10, b01, b00);
+                            hrr_rt(gc, R, sel3(ax, sd.rab[0], sd.rab[1], sd.rab[2]),
+                                   sel3(ax, ot.rab[0], ot.rab[1], ot.rab[2]));
+                        }
+                        __syncthreads();
+                        for (int f = lane; f < nf; f += 64) {
+                            const double Wk = W[f];
+                            const int fa = f % nfa, t1 = f / nfa, fb = t1 % nfb, t2 = t1 / nfb, fc = t2 % nfc,
+                                      fd = t2 / nfc;
+                            int bx[3], n0[3], n1[3];
+#pragma unroll
+                            for (int xx = 0; xx < 3; xx++) {
+                                const int ia = HCART[la][fa][xx], ib = HCART[lb][fb][xx], ic = HCART[lc][fc][xx],
+                                          id = HCART[ld][fd][xx];
+                                const int xa = side == 0 ? ia : ic, xb = side == 0 ? ib : id;
+                                const int ya = side == 0 ? ic : ia, yb = side == 0 ? id : ib;
+                                bx[xx] = xa + ya * R.dk + yb * R.dl + xb * R.dj;
+                                n0[xx] = xa;
+                                n1[xx] = xb;
+                            }
+                            for (int c = 0; c < nb; c++) {
+                                const double *G0 = g + (3 * c) * R.gs, *G1 = G0 + R.gs, *G2 = G1 + R.gs;
+                                const double I[3] = {G0[bx[0]], G1[bx[1]], G2[bx[2]]};
+                                val += Wk * I[0] * I[1] * I[2];
+#pragma unroll
+                                for (int xx = 0; xx < 3; xx++) {
+                                    const double *Gx = xx == 0 ? G0 : (xx == 1 ? G1 : G2);
+                                    const double o1 = Wk * I[(xx + 1) % 3] * I[(xx + 2) % 3];
+                                    double d0 = 2.0 * al0 * Gx[bx[xx] + 1];
+                                    if (n0[xx]) d0 -= n0[xx] * Gx[bx[xx] - 1];
+                                    double d1 = 2.0 * al1 * Gx[bx[xx] + R.dj];
+                                    if (n1[xx]) d1 -= n1[xx] * Gx[bx[xx] - R.dj];
+                                    gx0[xx] += d0 * o1;
+                                    gx1[xx] += d1 * o1;
+                                    e0 -= Gx[bx[xx] + 2] * o1;
+                                    e1 -= Gx[bx[xx] + 2 * R.dj] * o1;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+                e0 = wave_sum64(e0);
+                e1 = wave_sum64(e1);
+                val = wave_sum64(val);
+                if (lane == 0) {
+                    const int2 ij = pp_idx[sd.off + o];
+                    const int k0 = si0.y + ij.x, k1 = si1.y + ij.y;
+                    atomicAdd(&dprim[2 * k0], e0);
+                    atomicAdd(&dprim[2 * k1], e1);
+                    atomicAdd(&dprim[2 * k0 + 1], val / coef[k0]);
+                    atomicAdd(&dprim[2 * k1 + 1], val / coef[k1]);
+                }
+            }
+        }
+#pragma unroll
+        for (int xx = 0; xx < 3; xx++) {
+            const double v0 = wave_sum64(gx0[xx]);
+            const double v1 = wave_sum64(gx1[xx]);
+            if (lane == 0) {
+                atomicAdd(&gatom[3 * si0.x + xx], v0);
+                atomicAdd(&gatom[3 * si1.x + xx], v1);
+            }
+        }
+    }
+}
+
+hipError_t launch_hl_grad(int la, int lb, int lc, int ld, int side, const JKTask *tasks, int ntask, const DevBasis &B,
+                          const int2 *pp_idx, const int2 *shell_info, const double *coef, const double *D, int nao,
+                          double *gatom, double *dprim, double *scr, size_t scr_len, hipStream_t st)
+{
+    if (ntask <= 0) return hipSuccess;
+    if (side != 0 && side != 1) return hipErrorInvalidValue;
+    if (la < 0 || la > lb || lb > HL_LMAX || lc < 0 || lc > ld || ld > HL_LMAX) return hipErrorInvalidValue;
+    const HLShape R = side == 0 ? hl_grad_shape(la, lb, lc, ld) : hl_grad_shape(lc, ld, la, lb);
+    if (R.nroots > RYS_NMAX || 3 * R.rb > 64) return hipErrorInvalidValue;
+    const HLShape O = hl_shape(la, lb, lc, ld);  // outputs: the unraised class
+    const int grid = hl_groups(O, ntask, scr_len);
+    if (grid <= 0) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * 3 * (size_t)R.rb * R.gs;
+    hipLaunchKernelGGL(hl_grad_kernel, dim3(grid), dim3(64), lds, st, R, la, lb, lc, ld, side, tasks, ntask, B,
+                       pp_idx, shell_info, coef, D, nao, gatom, dprim, scr);
+    return hipGetLastError();
 }
 
 static int hl_groups(const HLShape &S, int ntask, size_t scr_len)

@@ -92,7 +92,7 @@ void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, 
 static void deriv_gpu(const Int1eInput &in, int typ, const double *W, double *gatom, double *denv, size_t env_l,
                       double *dSout, size_t s1, size_t L)
 {
-    const Staged g = stage(in, 2, "one-electron derivatives cover l <= 2");
+    const Staged g = stage(in, 4, "one-electron derivatives cover l <= 4");
     const size_t nn = (size_t)in.nao * in.nao;
     const size_t na = in.charges.size();
     DevBuf<Shell1Dev> dsh(g.sh);
@@ -104,8 +104,8 @@ static void deriv_gpu(const Int1eInput &in, int typ, const double *W, double *ga
     if (gatom) ck(hipMemset(dga.p, 0, sizeof(double) * 3 * na), "hipMemset");
     if (denv && env_l) ck(hipMemset(dde.p, 0, sizeof(double) * env_l), "hipMemset");
     if (dSout && nn * L) ck(hipMemset(ddS.p, 0, sizeof(double) * nn * L), "hipMemset");
-    for (int li = 0; li < 3; li++)
-        for (int lj = 0; lj < 3; lj++)
+    for (int li = 0; li < 5; li++)
+        for (int lj = 0; lj < 5; lj++)
             ck(launch_int1e_deriv(li, lj, dpairs.p + g.first[li][lj], g.count[li][lj], dsh.p, denv_in.p, dnuc.p,
                                   (int)g.nuc.size(), in.nao, typ, dW.p, dga.p, dde.p, (long long)env_l, ddS.p,
                                   (long long)s1, (long long)L, 0),

@@ -379,6 +379,8 @@ hipError_t launch_int1e_deriv(int li, int lj, const int2 *pairs, int npair, cons
         return hipGetLastError();                                                                                      \
     }
     X(0, 0) X(0, 1) X(0, 2) X(1, 0) X(1, 1) X(1, 2) X(2, 0) X(2, 1) X(2, 2)
+    X(0, 3) X(1, 3) X(2, 3) X(3, 0) X(3, 1) X(3, 2) X(3, 3)
+    X(0, 4) X(1, 4) X(2, 4) X(3, 4) X(4, 0) X(4, 1) X(4, 2) X(4, 3) X(4, 4)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -89,6 +89,18 @@ struct ListTask {
     long long out;
 };
 
+// Per-task precision choice of the mixed Fock kernels (kernels.hip,
+// launch_mix_bounds): a task runs FP32 recurrences when
+// Q_b max(Dpair[b] Q[ket0], Wqd[ket0]) < tau_mix, an upper bound of every
+// quartet's Q_ab Q_cd max(|D| over its six shell blocks); prec[t] receives
+// 0 (FP64) or 1 (FP32).
+struct MixArgs {
+    const double *Dpair;
+    const double *Wqd;
+    double tau_mix;
+    unsigned char *prec;
+};
+
 // Ket-side Coulomb slabs: instead of one global atomic per quartet into
 // M[c][d], a J/K launch stores each quartet's ket-pair Coulomb block in the
 // slab of its task, J[jbase + (t - tfirst)*64*NFCD + e*64 + lane]
@@ -108,25 +120,34 @@ bool class_supported(int la, int lb, int lc, int ld);
 // tasks whose flag bit is set in `accept` (flags == nullptr: all tasks).
 // counter: one int per launch, zero before the launch (launch_zero_counters)
 // jslab: slab base of this launch (null: ket Coulomb block by global atomics);
-// slab of task t = jslab + (t - tfirst)*64*NFC*NFD with t indexing `tasks`
+// slab of task t = jslab + (t - tfirst)*64*NFC*NFD with t indexing `tasks`.
+// The flat kernel is FP64 only (fp32 is ignored).
 hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tasks, int ntask, int *counter,
                      const unsigned char *flags, int accept, const DevBasis &B, const double *D, double *M, int nao,
                      double *jslab, int tfirst, hipStream_t st);
 hipError_t launch_zero_counters(int *p, int n, hipStream_t st);
+hipError_t launch_zero_doubles(double *p, size_t n, hipStream_t st);
 bool class_has_fp32(int la, int lb, int lc, int ld);
 // grouped form (exchange rows of the bra in LDS); jk_group_lds gives its LDS
 // bytes per workgroup, (size_t)-1 for classes without a grouped kernel
 // (drow: with the bra's density rows staged in LDS as well)
-size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao, bool drow);
-hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
+// mixed: the form that picks FP32 recurrences per task (MixArgs)
+size_t jk_group_lds(int la, int lb, int lc, int ld, int mixed, int nao, bool drow);
+hipError_t launch_jk_group(int la, int lb, int lc, int ld, int mixed, const JKGroup *groups, int ngroup,
                            const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
                            int accept, const DevBasis &B, const double *D, double *M, int nao, double *jslab,
-                           int tfirst, bool drow, hipStream_t st);
+                           int tfirst, bool drow, const MixArgs &mx, hipStream_t st);
+// per-build bounds of the mixed kernels from Dsh (Drow: nsh, Dpair, qd, Wqd: npair)
+hipError_t launch_mix_bounds(const double *Dsh, int nsh, const DevBasis &B, int npair, double *Drow, double *Dpair,
+                             double *qd, double *Wqd, hipStream_t st);
+// shell-pair block maxima of |D| (Dsh[i*nsh + j], virtual segmented shells)
+hipError_t launch_dsh(const double *D, const int *ao_off, int nsh, int nao, double *Dsh, hipStream_t st);
 // sum the Coulomb slabs of every item's tasks into M (tasks: global task array)
 hipError_t launch_jreduce(const JRedItem *items, int nitem, int max_nfcd, const int *item_tasks, const JKTask *tasks,
                           const unsigned char *flags, int accept, const double *J, const DevBasis &B, double *M,
                           int nao, hipStream_t st);
-// per-build task classification (shell-block max |D|, then flags and group masks)
+// per-build task classification from the shell-block maxima Dsh (launch_dsh
+// first): flags (0 FP64, 1 below tau_mix, 2 below tau_skip) and group masks
 hipError_t launch_classify(const JKTask *tasks, int ntask, const int *task_group, int ngroup, const DevBasis &B,
                            const double *D, const int *ao_off, int nsh, int nao, double *Dsh, double tau_mix,
                            double tau_skip, unsigned char *flags, unsigned int *gmask_w, unsigned char *gmask,

@@ -17,6 +17,9 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #include "eri_core.h"
 #include "kernels.h"
@@ -346,9 +349,23 @@ __host__ __device__ constexpr int RysLdsDoubles()
     return (int)((RysLds<LA, LB, LC, LD, T>::LEN * sizeof(T) + 15) / 16 * 2);
 }
 
+// LDS doubles of the Rys tables a group kernel stages: the FP64 block, and
+// with MIXED the FP32 block behind it
+template <int LA, int LB, int LC, int LD, bool MIXED>
+__host__ __device__ constexpr int GroupTabDoubles()
+{
+    return RysLdsDoubles<LA, LB, LC, LD, double>() + (MIXED ? RysLdsDoubles<LA, LB, LC, LD, float>() : 0);
+}
+
 // gmask: per group, bit f set when the group holds a task with flag f.
 // DROW: the bra's density rows (NFA + NFB rows of nao) are staged in LDS
 // beside the exchange rows, so the exchange terms read D from LDS.
+// MIXED: each task runs FP32 recurrences when its bound
+// Q_b max(Dpair[b] Q[ket0], Wqd[ket0]) (MixArgs, wavefront-uniform scalar
+// loads) is below tau_mix, else FP64, and records the choice in prec[t]; the
+// FP32 path has its own Rys table block in LDS.  Each wavefront loads the
+// classification flags of its tasks of a group at once (lane j: its j-th
+// task), so the per-task skip test does not wait on memory.
 // Minimum wavefronts per SIMD asked of the compiler per class: 4 for (01|00)
 // (153 VGPRs would allow 3; 128 with 5 spilled), 5 for (00|00) (102 -> 96
 // VGPRs, no spill); no request elsewhere (59 spills for 3 in (01|01)) nor
@@ -361,29 +378,30 @@ struct GroupWaves {
                                                                                 : 1;
 };
 
-template <int LA, int LB, int LC, int LD, typename T, bool DROW>
+template <int LA, int LB, int LC, int LD, bool MIXED, bool DROW>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GroupWaves<LA, LB, LC, LD, DROW>::value))) void
-jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
-                                                       const JKTask *__restrict__ tasks, int *__restrict__ counter,
-                                                       const unsigned char *__restrict__ flags,
-                                                       const unsigned char *__restrict__ gmask, int accept, DevBasis B,
-                                                       const double *__restrict__ D, double *__restrict__ M, int nao,
-                                                       double *__restrict__ jslab, int tfirst,
-                                                       const PrimPair *__restrict__ pp)
+jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup, const JKTask *__restrict__ tasks,
+                int *__restrict__ counter, const unsigned char *__restrict__ flags,
+                const unsigned char *__restrict__ gmask, int accept, DevBasis B, const double *__restrict__ D,
+                double *__restrict__ M, int nao, double *__restrict__ jslab, int tfirst,
+                const PrimPair *__restrict__ pp, MixArgs mx)
 {
     using C = Class<LA, LB, LC, LD>;
-    using R = RysLds<LA, LB, LC, LD, T>;
+    using R64 = RysLds<LA, LB, LC, LD, double>;
+    using R32 = RysLds<LA, LB, LC, LD, float>;
     // the primitive-pair records through a restrict-qualified argument:
     // read-only and unaliased, so wavefront-uniform (bra) records can be
     // fetched with scalar loads
     B.pp = pp;
     extern __shared__ __align__(16) double lds[];
-    T *rt = (T *)lds;                                       // Rys table block (16-byte aligned)
-    double *mab_l = lds + RysLdsDoubles<LA, LB, LC, LD, T>(); // bra-bra block
-    double *Krow = mab_l + C::NFA * C::NFB;                 // (NFA + NFB) rows of nao
-    double *Drow = Krow + (C::NFA + C::NFB) * nao;          // density rows (DROW)
+    double *rt = lds;                                                        // FP64 Rys table block
+    float *rt32 = (float *)(lds + RysLdsDoubles<LA, LB, LC, LD, double>()); // FP32 block (MIXED)
+    double *mab_l = lds + GroupTabDoubles<LA, LB, LC, LD, MIXED>();         // bra-bra block
+    double *Krow = mab_l + C::NFA * C::NFB;                                 // (NFA + NFB) rows of nao
+    double *Drow = Krow + (C::NFA + C::NFB) * nao;                          // density rows (DROW)
     __shared__ int g_s;
-    rys_stage<T, R::N, R::WONLY>(rt);
+    rys_stage<double, R64::N, R64::WONLY>(rt);
+    if constexpr (MIXED) rys_stage<float, R32::N, R32::WONLY>(rt32);
     const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int nrow = (C::NFA + C::NFB) * nao;
     const size_t n = (size_t)nao;
@@ -415,10 +433,30 @@ jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
         double mab[C::NFA * C::NFB];
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) mab[e] = 0.0;
-        for (int t = G.first + wave; t < G.first + G.count; t += 4) {
-            if (flags && !((accept >> flags[t]) & 1)) continue;
+        // this wavefront's tasks of the group are G.first + wave + 4 j (j < 64:
+        // groups hold at most 256 tasks); lane j holds the flag of the j-th
+        int myflag = 0;
+        if (flags && G.first + wave + 4 * lane < G.first + G.count) myflag = flags[G.first + wave + 4 * lane];
+        double qdb = 0.0, qb = 0.0;
+        if constexpr (MIXED) {
+            qb = B.pair_rb[G.bra].w;
+            qdb = mx.Dpair[G.bra];
+        }
+        for (int t = G.first + wave, j = 0; t < G.first + G.count; t += 4, j++) {
+            const int fl = flags ? __builtin_amdgcn_readlane(myflag, j) : 0;
+            if (!((accept >> fl) & 1)) continue;
+            const JKTask tk = tasks[t];
             double *js = jslab ? jslab + (size_t)(t - tfirst) * (64 * C::NFC * C::NFD) : nullptr;
-            jk_task_rows<LA, LB, LC, LD, T, DROW>(tasks[t], bra, B, D, M, nao, lane, rt, Krow, mab, js, Drow);
+            bool f32 = false;
+            if constexpr (MIXED) {
+                const int k0 = __builtin_amdgcn_readfirstlane(tk.ket0);
+                f32 = qb * fmax(qdb * B.pair_rb[k0].w, mx.Wqd[k0]) < mx.tau_mix;
+                if (lane == 0) mx.prec[t] = f32 ? 1 : 0;
+            }
+            if (f32)
+                jk_task_rows<LA, LB, LC, LD, float, DROW>(tk, bra, B, D, M, nao, lane, rt32, Krow, mab, js, Drow);
+            else
+                jk_task_rows<LA, LB, LC, LD, double, DROW>(tk, bra, B, D, M, nao, lane, rt, Krow, mab, js, Drow);
         }
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) {
@@ -440,49 +478,64 @@ jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup,
     }
 }
 
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, bool MIXED>
 static size_t group_lds_bytes(int nao, bool drow)
 {
     using C = Class<LA, LB, LC, LD>;
-    return sizeof(double) * ((size_t)RysLdsDoubles<LA, LB, LC, LD, T>() + C::NFA * C::NFB +
+    return sizeof(double) * ((size_t)GroupTabDoubles<LA, LB, LC, LD, MIXED>() + C::NFA * C::NFB +
                              (size_t)(C::NFA + C::NFB) * nao * (drow ? 2 : 1));
 }
 
-template <int LA, int LB, int LC, int LD, typename T, bool DROW>
+// Persistent-grid size (resident workgroups per CU x CUs) of a kernel at a
+// given dynamic LDS size on the current device, computed once per (kernel,
+// device, LDS size) under a lock, so concurrent callers and several devices
+// each get their own.
+static int persistent_grid(const void *kernel, int threads, size_t lds, const char *what)
+{
+    static std::mutex mu;
+    static std::map<std::tuple<const void *, int, size_t>, int> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    const auto key = std::make_tuple(kernel, dev, lds);
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int ncu = 0, per = 0;
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, lds);
+    const int grid = std::max(1, ncu) * std::max(1, per);
+    cache[key] = grid;
+    if (getenv("LIBRINT_DEBUG"))
+        fprintf(stderr, "librint: %s lds %zu blocks/cu %d grid %d (device %d)\n", what, lds, per, grid, dev);
+    return grid;
+}
+
+template <int LA, int LB, int LC, int LD, bool MIXED, bool DROW>
 static hipError_t launch_jk_group_d(const JKGroup *groups, int ngroup, const JKTask *tasks, int *counter,
                                     const unsigned char *flags, const unsigned char *gmask, int accept,
                                     const DevBasis &B, const double *D, double *M, int nao, double *jslab,
-                                    int tfirst, hipStream_t st)
+                                    int tfirst, const MixArgs &mx, hipStream_t st)
 {
-    const size_t lds = group_lds_bytes<LA, LB, LC, LD, T>(nao, DROW);
-    static int grid = 0, grid_nao = -1;
-    if (grid == 0 || grid_nao != nao) {
-        int dev = 0, ncu = 0, per = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_group_kernel<LA, LB, LC, LD, T, DROW>, 256, lds);
-        grid = std::max(1, ncu) * std::max(1, per);
-        grid_nao = nao;
-        if (getenv("LIBRINT_DEBUG"))
-            fprintf(stderr, "librint: jk_group<%d%d|%d%d>%s%s lds %zu blocks/cu %d grid %d\n", LA, LB, LC, LD,
-                    sizeof(T) == 4 ? " fp32" : "", DROW ? " drow" : "", lds, per, grid);
-    }
+    const size_t lds = group_lds_bytes<LA, LB, LC, LD, MIXED>(nao, DROW);
+    char what[64];
+    snprintf(what, sizeof what, "jk_group<%d%d|%d%d>%s%s", LA, LB, LC, LD, MIXED ? " mixed" : "", DROW ? " drow" : "");
+    const int grid = persistent_grid((const void *)jk_group_kernel<LA, LB, LC, LD, MIXED, DROW>, 256, lds, what);
     const int g = std::min(grid, ngroup);
-    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, T, DROW>), dim3(g), dim3(256), lds, st, groups, ngroup, tasks,
-                       counter, flags, gmask, accept, B, D, M, nao, jslab, tfirst, B.pp);
+    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, MIXED, DROW>), dim3(g), dim3(256), lds, st, groups, ngroup,
+                       tasks, counter, flags, gmask, accept, B, D, M, nao, jslab, tfirst, B.pp, mx);
     return hipGetLastError();
 }
 
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, bool MIXED>
 static hipError_t launch_jk_group_t(const JKGroup *groups, int ngroup, const JKTask *tasks, int *counter,
                                     const unsigned char *flags, const unsigned char *gmask, int accept,
                                     const DevBasis &B, const double *D, double *M, int nao, double *jslab,
-                                    int tfirst, bool drow, hipStream_t st)
+                                    int tfirst, bool drow, const MixArgs &mx, hipStream_t st)
 {
-    return drow ? launch_jk_group_d<LA, LB, LC, LD, T, true>(groups, ngroup, tasks, counter, flags, gmask, accept, B,
-                                                              D, M, nao, jslab, tfirst, st)
-                : launch_jk_group_d<LA, LB, LC, LD, T, false>(groups, ngroup, tasks, counter, flags, gmask, accept, B,
-                                                               D, M, nao, jslab, tfirst, st);
+    return drow ? launch_jk_group_d<LA, LB, LC, LD, MIXED, true>(groups, ngroup, tasks, counter, flags, gmask, accept,
+                                                                  B, D, M, nao, jslab, tfirst, mx, st)
+                : launch_jk_group_d<LA, LB, LC, LD, MIXED, false>(groups, ngroup, tasks, counter, flags, gmask,
+                                                                   accept, B, D, M, nao, jslab, tfirst, mx, st);
 }
 
 // ------------------------------------------------------ team J/K kernel --
@@ -894,13 +947,70 @@ __global__ void pack_gmask_kernel(const unsigned int *__restrict__ w, unsigned c
     if (i < n) b[i] = (unsigned char)w[i];
 }
 
+// Mixed-precision bounds per build (from Dsh):
+//   Drow[s]  = max_j Dsh[s][j]
+//   Dpair[p] = max(Drow[sa], Drow[sb])   >= |D| over every block a quartet
+//              with pair p reads on p's side (ab, ac, ad, bc, bd)
+//   Wqd[k]   = max over k' in [k, k + 64) of Q[k'] Dpair[k']
+// so that a task (bra b, kets [ket0, ket0 + nket), nket <= 64, Q descending)
+// has every quartet bound Q_b Q_k max(Dsh of its six blocks) below
+//   Q_b max(Dpair[b] Q[ket0], Wqd[ket0]).
+__global__ void drow_kernel(const double *__restrict__ Dsh, int nsh, double *__restrict__ Drow)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsh) return;
+    double m = 0.0;
+    for (int j = 0; j < nsh; j++) m = fmax(m, Dsh[(size_t)s * nsh + j]);
+    Drow[s] = m;
+}
+
+__global__ void dpair_kernel(const int4 *__restrict__ pair_info, const double4 *__restrict__ pair_rb, int npair,
+                             const double *__restrict__ Drow, double *__restrict__ Dpair, double *__restrict__ qd)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npair) return;
+    const int4 pi = pair_info[p];
+    const double d = fmax(Drow[pi.x], Drow[pi.y]);
+    Dpair[p] = d;
+    qd[p] = d * pair_rb[p].w;
+}
+
+__global__ void wqd_kernel(const double *__restrict__ qd, int npair, double *__restrict__ Wqd)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= npair) return;
+    double m = 0.0;
+    const int e = min(npair, k + 64);
+    for (int j = k; j < e; j++) m = fmax(m, qd[j]);
+    Wqd[k] = m;
+}
+
+hipError_t launch_mix_bounds(const double *Dsh, int nsh, const DevBasis &B, int npair, double *Drow, double *Dpair,
+                             double *qd, double *Wqd, hipStream_t st)
+{
+    if (nsh <= 0 || npair <= 0) return hipSuccess;
+    hipLaunchKernelGGL(drow_kernel, dim3((nsh + 255) / 256), dim3(256), 0, st, Dsh, nsh, Drow);
+    hipLaunchKernelGGL(dpair_kernel, dim3((npair + 255) / 256), dim3(256), 0, st, B.pair_info, B.pair_rb, npair, Drow,
+                       Dpair, qd);
+    hipLaunchKernelGGL(wqd_kernel, dim3((npair + 255) / 256), dim3(256), 0, st, qd, npair, Wqd);
+    return hipGetLastError();
+}
+
+hipError_t launch_dsh(const double *D, const int *ao_off, int nsh, int nao, double *Dsh, hipStream_t st)
+{
+    const long long nn = (long long)nsh * nsh;
+    hipLaunchKernelGGL(dsh_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, D, ao_off, nsh, nao, Dsh);
+    return hipGetLastError();
+}
+
 hipError_t launch_classify(const JKTask *tasks, int ntask, const int *task_group, int ngroup, const DevBasis &B,
                            const double *D, const int *ao_off, int nsh, int nao, double *Dsh, double tau_mix,
                            double tau_skip, unsigned char *flags, unsigned int *gmask_w, unsigned char *gmask,
                            hipStream_t st)
 {
-    const long long nn = (long long)nsh * nsh;
-    hipLaunchKernelGGL(dsh_kernel, dim3((unsigned)((nn + 255) / 256)), dim3(256), 0, st, D, ao_off, nsh, nao, Dsh);
+    (void)D;
+    (void)ao_off;
+    (void)nao;
     if (ngroup > 0)
         hipLaunchKernelGGL(zero_u32_kernel, dim3((ngroup + 255) / 256), dim3(256), 0, st, gmask_w, ngroup);
     if (ntask > 0)
@@ -950,17 +1060,9 @@ static hipError_t launch_jk_persistent(const JKTask *tasks, int ntask, int *coun
                                        int accept, const DevBasis &B, const double *D, double *M, int nao,
                                        double *jslab, int tfirst, hipStream_t st)
 {
-    static int grid = 0;
-    if (grid == 0) {
-        int dev = 0, ncu = 0, per = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, jk_kernel<LA, LB, LC, LD, T>, 256, 0);
-        grid = std::max(1, ncu) * std::max(1, per);
-        if (getenv("LIBRINT_DEBUG"))
-            fprintf(stderr, "librint: jk<%d%d|%d%d>%s blocks/cu %d grid %d\n", LA, LB, LC, LD,
-                    sizeof(T) == 4 ? " fp32" : "", per, grid);
-    }
+    char what[48];
+    snprintf(what, sizeof what, "jk<%d%d|%d%d>", LA, LB, LC, LD);
+    const int grid = persistent_grid((const void *)jk_kernel<LA, LB, LC, LD, T>, 256, 0, what);
     const int g = std::min(grid, (ntask + 3) / 4);
     hipLaunchKernelGGL((jk_kernel<LA, LB, LC, LD, T>), dim3(g), dim3(256), 0, st, tasks, ntask, counter, flags, accept, B,
                        D, M, nao, jslab, tfirst);
@@ -971,6 +1073,20 @@ __global__ void zero_ints_kernel(int *p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+__global__ void zero_doubles_kernel(double *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+hipError_t launch_zero_doubles(double *p, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3(blocks), dim3(256), 0, st, p, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_zero_counters(int *p, int n, hipStream_t st)
@@ -996,10 +1112,8 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tas
     if (ntask == 0) return hipSuccess;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
-        return fp32 ? launch_jk_persistent<a, b, c, d, float>(tasks, ntask, counter, flags, accept, B, D, M, nao,       \
-                                                              jslab, tfirst, st)                                       \
-                    : launch_jk_persistent<a, b, c, d, double>(tasks, ntask, counter, flags, accept, B, D, M, nao,      \
-                                                               jslab, tfirst, st);
+        return launch_jk_persistent<a, b, c, d, double>(tasks, ntask, counter, flags, accept, B, D, M, nao, jslab,     \
+                                                        tfirst, st);
     LR_REG_CLASSES(X)
 #undef X
     if (fp32 || jslab) return hipErrorInvalidValue;
@@ -1014,28 +1128,29 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tas
     return hipErrorInvalidValue;
 }
 
-size_t jk_group_lds(int la, int lb, int lc, int ld, int fp32, int nao, bool drow)
+size_t jk_group_lds(int la, int lb, int lc, int ld, int mixed, int nao, bool drow)
 {
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
-        return fp32 ? group_lds_bytes<a, b, c, d, float>(nao, drow) : group_lds_bytes<a, b, c, d, double>(nao, drow);
+        return mixed ? group_lds_bytes<a, b, c, d, true>(nao, drow) : group_lds_bytes<a, b, c, d, false>(nao, drow);
     LR_REG_CLASSES(X)
 #undef X
     return (size_t)-1;
 }
 
-hipError_t launch_jk_group(int la, int lb, int lc, int ld, int fp32, const JKGroup *groups, int ngroup,
+hipError_t launch_jk_group(int la, int lb, int lc, int ld, int mixed, const JKGroup *groups, int ngroup,
                            const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
                            int accept, const DevBasis &B, const double *D, double *M, int nao, double *jslab,
-                           int tfirst, bool drow, hipStream_t st)
+                           int tfirst, bool drow, const MixArgs &mx, hipStream_t st)
 {
     if (ngroup == 0) return hipSuccess;
+    if (mixed && (!mx.Dpair || !mx.Wqd || !mx.prec)) return hipErrorInvalidValue;
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
-        return fp32 ? launch_jk_group_t<a, b, c, d, float>(groups, ngroup, tasks, counter, flags, gmask, accept, B, D, \
-                                                           M, nao, jslab, tfirst, drow, st)                            \
-                    : launch_jk_group_t<a, b, c, d, double>(groups, ngroup, tasks, counter, flags, gmask, accept, B,   \
-                                                            D, M, nao, jslab, tfirst, drow, st);
+        return mixed ? launch_jk_group_t<a, b, c, d, true>(groups, ngroup, tasks, counter, flags, gmask, accept, B, D,  \
+                                                           M, nao, jslab, tfirst, drow, mx, st)                        \
+                     : launch_jk_group_t<a, b, c, d, false>(groups, ngroup, tasks, counter, flags, gmask, accept, B,   \
+                                                            D, M, nao, jslab, tfirst, drow, mx, st);
     LR_REG_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;

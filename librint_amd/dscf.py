@@ -15,8 +15,9 @@ import numpy as np
 from ._bindings import library
 from . import utils
 
-# The adjoint kernel covers cartesian shells with l <= 2 (the GPU classes).
-GRAD_LMAX = 2
+# The adjoint kernels cover cartesian shells with l <= 4 (the reference's
+# memory-safe domain, reference python/librint/dscf.py:11).
+GRAD_LMAX = 4
 
 
 def _require_grad_domain(mol):

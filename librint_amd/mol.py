@@ -327,6 +327,25 @@ def h2o_fg():
     return build(atoms, norm, "h2o_fg")
 
 
+def h2o_gc():
+    """H2O with generally contracted shells (nctr = 2 on shared primitives):
+    on O an s pair over the six STO-3G s exponents, a p pair over the three
+    sp exponents and a d (zeta 0.8); on H an s pair over the STO-3G
+    exponents and a p (zeta 0.75).  24 cartesian functions.  Parity molecule
+    for the general-contraction paths (not a published basis)."""
+    atoms = [(8, _ang(WATER_INTERNAL[0])), (1, _ang(WATER_INTERNAL[1])), (1, _ang(WATER_INTERNAL[2]))]
+    os_ = [130.70932, 23.808861, 6.4436083, 5.0331513, 1.1695961, 0.380389]
+    os_c = np.array([[0.15432897, 0.53532814, 0.44463454, 0.02, 0.01, 0.005],
+                     [-0.005, -0.02, -0.09996723, 0.39951283, 0.70011547, 0.3]]).T
+    op = [5.0331513, 1.1695961, 0.380389]
+    op_c = np.array([[0.15591627, 0.60768372, 0.39195739], [0.1, -0.4, 1.0]]).T
+    hs = [3.42525091, 0.62391373, 0.1688554]
+    hs_c = np.array([[0.15432897, 0.53532814, 0.44463454], [0.3, -0.5, 1.0]]).T
+    basis = {8: [(0, os_, os_c), (1, op, op_c), (2, [0.8], [1.0])],
+             1: [(0, hs, hs_c), (1, [0.75], [1.0])]}
+    return build(atoms, basis, "h2o_gc", normalized=False)
+
+
 def h2o_631gstar_like():
     """Small d-shell test molecule: H2O with the carbon 6-31G* d shell on O
     (parameters only; used for d-shell parity, not a published basis)."""

@@ -37,7 +37,7 @@
 #define BAS_SLOTS 8
 #define LMAX 4
 #define NFMAX 15
-#define NRMAX 9
+#define NRMAX 10
 #define GSIZE_MAX ((LMAX + 1) * (LMAX + 1) * (2 * LMAX + 1) * (2 * LMAX + 1))
 
 typedef long double ld;

@@ -85,7 +85,7 @@ def main():
     out = {"boys": [], "rys": []}
     for x in XS:
         out["boys"].append({"x": x, "F": [float(boys(m, mp.mpf(x))) for m in range(0, 19)]})
-    for n in range(1, 10):
+    for n in range(1, 11):
         for x in XS:
             if x > 2000:
                 continue

@@ -141,17 +141,3 @@ def test_scf_f_g_vs_oracle(gpu, mfg):
     E = scf.scf(mfg, imax=200, conv=1e-8)
     Er = refcpu.scf(mfg, imax=200, conv=1e-8)
     assert abs(E - Er) < 1e-10, (E, Er)
-
-
-def test_gradient_rejects_f_g(gpu, mfg):
-    """Analytic gradients cover l <= 2: the Python entry refuses the basis,
-    and the C entry reports an error instead of computing."""
-    from librint_amd import autograd, dscf
-    from librint_amd.context import FockContext
-    _say("gradient f/g rejected")
-    n = mfg.nao()
-    with pytest.raises(ValueError):
-        dscf.grad(mfg, np.eye(n))
-    ctx = FockContext(mfg)
-    with pytest.raises(RuntimeError, match="l <= 2"):
-        autograd.gradient(ctx, np.eye(n))

@@ -48,7 +48,7 @@ def _e1(mol, P):
 
 def _mols():
     from librint_amd import mol
-    return {"h2o_d": mol.h2o_631gstar_like, "w2": lambda: mol.water_cluster(2)}
+    return {"h2o_d": mol.h2o_631gstar_like, "w2": lambda: mol.water_cluster(2), "h2o_fg": mol.h2o_fg}
 
 
 def _density(n, seed):
@@ -61,7 +61,7 @@ def _coord_slots(mol):
     return [(a, d, int(mol._atm[a][1]) + d) for a in range(len(mol._atm)) for d in range(3)]
 
 
-@pytest.mark.parametrize("name", ["h2o_d", "w2"])
+@pytest.mark.parametrize("name", ["h2o_d", "w2", "h2o_fg"])
 def test_dR_fixed_density_vs_fd(gpu, name):
     """Exponent and coefficient derivatives of E2 (dR_c) at a fixed density."""
     from librint_amd import dscf
@@ -86,7 +86,7 @@ def test_dR_fixed_density_vs_fd(gpu, name):
     _say("  worst rel err %.2e" % worst)
 
 
-@pytest.mark.parametrize("name", ["h2o_d", "w2"])
+@pytest.mark.parametrize("name", ["h2o_d", "w2", "h2o_fg"])
 def test_nuclear_fixed_density_vs_fd(gpu, name):
     """Nuclear-coordinate derivatives of E2 (adjoint kernel) and of tr(P H)
     (host, including the point-charge positions) at a fixed density."""
@@ -188,6 +188,39 @@ def test_rhf_gradient_vs_fd_water_dimer(gpu):
     m = _mols()["w2"]()
     _say("RHF gradient (H2O)2 nao", m.nao())
     _check_rhf_fd(m, _coord_slots(m), autograd.exponent_slots(m), 1e-6)
+
+
+def test_rhf_gradient_vs_fd_f_g(gpu):
+    """Converged-RHF gradient with f and g shells (a contracted f and a g on
+    O, d on H; up to 10 Rys roots in the adjoint): every nuclear coordinate,
+    exponent and contraction coefficient against 5-point FD (coordinates,
+    exponents) and central FD (coefficients) of re-converged energies."""
+    from librint_amd import autograd, dscf
+    m = _mols()["h2o_fg"]()
+    _say("RHF gradient H2O f/g nao", m.nao())
+    _check_rhf_fd(m, _coord_slots(m), autograd.exponent_slots(m), 1e-6)
+    E0, P0, it, ctx = autograd.rhf(m, conv=1e-10)
+    ctx.close()
+    s1, s2 = dscf.split(m._bas)
+    da = dscf.danalyticalf(m, P0)
+    exps = set(autograd.exponent_slots(m).tolist())
+    env0 = m._env.copy()
+    worst, n = 0.0, 0
+    for k in range(s1, s2):
+        if k in exps:
+            continue
+        h = 1e-5 * max(abs(env0[k]), 1e-2)
+        m._env[k] = env0[k] + h
+        ep = _rhf_energy(m, P0)
+        m._env[k] = env0[k] - h
+        em = _rhf_energy(m, P0)
+        m._env[:] = env0
+        fd = (ep - em) / (2 * h)
+        worst = max(worst, abs(fd - da[k - s1]))
+        assert abs(fd - da[k - s1]) < 1e-6, (k, fd, da[k - s1])
+        n += 1
+    assert n > 0
+    _say("  coefficients: %d, worst abs err %.2e" % (n, worst))
 
 
 def test_rhf_gradient_reference_entries_agree(gpu):

@@ -54,7 +54,7 @@ def test_rys_vs_mpmath():
         assert np.abs(w - e["w"]).max() < 5e-16, (e["n"], e["x"])
 
 
-@pytest.mark.parametrize("n", [1, 3, 5, 9])
+@pytest.mark.parametrize("n", [1, 3, 5, 9, 10])
 def test_rys_moments(n):
     # sum_r w_r v_r^k = F_k(x) for k < 2n (Gauss rule exactness)
     for x in [0.0, 0.3, 2.0, 9.0, 27.0, 44.0, 70.0, 120.0]:
@@ -188,6 +188,34 @@ def test_int1e_vs_textbook_f_g_shells(typ):
     m = mol.h2o_fg()
     R = refcpu.int1e(m, typ)
     off = np.cumsum([0] + [(int(b[1]) + 1) * (int(b[1]) + 2) // 2 for b in m._bas])
+    for i in range(m.nbas):
+        for j in range(m.nbas):
+            blk = tb.int1e_block(m, i, j, typ)
+            assert np.abs(blk - R[off[i]:off[i + 1], off[j]:off[j + 1]]).max() < 1e-13, (typ, i, j)
+
+
+def test_eri_vs_textbook_general_contraction():
+    """The oracle's nctr > 1 branch (hierarchical primitive -> contraction
+    loop of reference src/eri.rs:571-692) against the McMurchie-Davidson
+    restatement contracting each column separately, on H2O with pairs of
+    contractions over shared s and p primitives."""
+    from librint_amd import mol
+    m = mol.h2o_gc()
+    assert max(int(b[3]) for b in m._bas) == 2
+    # shells: O 0 s(x2), 1 p(x2), 2 d; H 3 s(x2), 4 p; H 5 s(x2), 6 p
+    for q in [(0, 0, 0, 0), (0, 1, 2, 3), (1, 1, 1, 1), (2, 0, 3, 0), (4, 5, 0, 1), (1, 3, 5, 2),
+              (6, 0, 1, 5), (3, 3, 5, 5)]:
+        a = tb.eri_block(m, q)
+        assert np.abs(a - refcpu.int2e_shell(m, q)).max() < 1e-13 * max(1.0, np.abs(a).max()), q
+
+
+@pytest.mark.parametrize("typ", [0, 1, 2])
+def test_int1e_vs_textbook_general_contraction(typ):
+    from librint_amd import mol
+    m = mol.h2o_gc()
+    R = refcpu.int1e(m, typ)
+    off = np.cumsum([0] + [(int(b[1]) + 1) * (int(b[1]) + 2) // 2 * int(b[3]) for b in m._bas])
+    assert off[-1] == m.nao() == 24
     for i in range(m.nbas):
         for j in range(m.nbas):
             blk = tb.int1e_block(m, i, j, typ)

@@ -54,8 +54,8 @@ def test_table_vs_mpmath():
 def test_table_vs_oracle_dense():
     from oracle import refcpu
     rng = np.random.default_rng(11)
-    for n in range(1, 10):
-        for x in np.concatenate([rng.uniform(0, 90, 60), [0.0, 1e-12, float(TAB[2][n]) - 1e-9, float(TAB[2][n])]]):
+    for n in range(1, 11):
+        for x in np.concatenate([rng.uniform(0, 95, 60), [0.0, 1e-12, float(TAB[2][n]) - 1e-9, float(TAB[2][n])]]):
             v, w = table_nodes(n, x)
             vr, wr = refcpu.rys_nodes(n, float(x))
             assert np.abs(v - vr).max() < 4e-16 and np.abs(w - wr).max() < 4e-16, (n, x)

@@ -79,19 +79,38 @@ def cart(l):
     return out
 
 
-def shell(mol, s):
+def shell(mol, s, ctr=0):
+    """(l, centre, exponents, coefficients of contraction ctr times the
+    l = 0, 1 normalisation factor)."""
     b = mol._bas[s]
     a = mol._atm[b[0]]
     R = mol._env[a[1]:a[1] + 3]
     l, nprim = int(b[1]), int(b[2])
     exps = mol._env[b[5]:b[5] + nprim]
-    coefs = mol._env[b[6]:b[6] + nprim] * FAC_SP.get(l, 1.0)
+    coefs = mol._env[b[6] + ctr * nprim:b[6] + (ctr + 1) * nprim] * FAC_SP.get(l, 1.0)
     return l, R, exps, coefs
 
 
 def eri_block(mol, shls):
-    """(ij|kl) block, shape (nfi, nfj, nfk, nfl), segmented shells."""
-    sh = [shell(mol, s) for s in shls]
+    """(ij|kl) block, shape (di, dj, dk, dl) with d = nf * nctr and the
+    component index fastest within each contraction (I = f + nf * c), the
+    layout of a generally contracted shell in cint2e_cart."""
+    nct = [int(mol._bas[s][3]) for s in shls]
+    nfs = [len(cart(int(mol._bas[s][1]))) for s in shls]
+    out = np.zeros([nf * nc for nf, nc in zip(nfs, nct)])
+    for c0 in range(nct[0]):
+        for c1 in range(nct[1]):
+            for c2 in range(nct[2]):
+                for c3 in range(nct[3]):
+                    blk = _eri_block_ctr(mol, shls, (c0, c1, c2, c3))
+                    out[c0 * nfs[0]:(c0 + 1) * nfs[0], c1 * nfs[1]:(c1 + 1) * nfs[1],
+                        c2 * nfs[2]:(c2 + 1) * nfs[2], c3 * nfs[3]:(c3 + 1) * nfs[3]] = blk
+    return out
+
+
+def _eri_block_ctr(mol, shls, ctr):
+    """(ij|kl) block of one contraction of each shell, shape (nfi, nfj, nfk, nfl)."""
+    sh = [shell(mol, s, c) for s, c in zip(shls, ctr)]
     (la, A, ea, ca), (lb, B, eb, cb), (lc, C, ec, cc), (ld, D, ed, cd) = sh
     fa, fb, fc, fd = cart(la), cart(lb), cart(lc), cart(ld)
     out = np.zeros((len(fa), len(fb), len(fc), len(fd)))
@@ -138,10 +157,22 @@ def _ovlp_1d(i, j, a, b, XAB):
 
 
 def int1e_block(mol, i, j, typ):
-    """One-electron block (nfi, nfj) by McMurchie-Davidson: typ 0 overlap,
-    1 kinetic -1/2 <i|lap|j>, 2 nuclear sum_C -Z_C <i|1/r_C|j>."""
-    li, A, ea, ca = shell(mol, i)
-    lj, B, eb, cb = shell(mol, j)
+    """One-electron block (nfi*nctr_i, nfj*nctr_j), I = f + nf * c, by
+    McMurchie-Davidson: typ 0 overlap, 1 kinetic -1/2 <i|lap|j>, 2 nuclear
+    sum_C -Z_C <i|1/r_C|j>."""
+    ni, nj = int(mol._bas[i][3]), int(mol._bas[j][3])
+    fi = len(cart(int(mol._bas[i][1])))
+    fj = len(cart(int(mol._bas[j][1])))
+    out = np.zeros((fi * ni, fj * nj))
+    for ci in range(ni):
+        for cj in range(nj):
+            out[ci * fi:(ci + 1) * fi, cj * fj:(cj + 1) * fj] = _int1e_block_ctr(mol, i, j, typ, ci, cj)
+    return out
+
+
+def _int1e_block_ctr(mol, i, j, typ, ci, cj):
+    li, A, ea, ca = shell(mol, i, ci)
+    lj, B, eb, cb = shell(mol, j, cj)
     fi, fj = cart(li), cart(lj)
     out = np.zeros((len(fi), len(fj)))
     atoms = [(abs(int(a[0])), mol._env[a[1]:a[1] + 3]) for a in mol._atm]
