@@ -129,6 +129,23 @@ int librint_fock(librint_ctx *ctx, const double *P, double *G);
  * Asynchronous: returns after enqueueing.  Returns 0 on success. */
 int librint_fock_device(librint_ctx *ctx, const double *P_dev, double *G_dev, void *stream);
 
+/* librint_fock_device without any device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The build's
+ * scratch (the nao*nao accumulator, the shell-block density maxima when
+ * density screening or mixed precision is on, the work counters and the
+ * block scratch of f/g classes) lives in a static 64 MiB device buffer of the
+ * library, one per GPU, held until the process ends; the caller passes P and
+ * G as for librint_fock_device.  One stream (`stream`) and no Coulomb slabs:
+ * the ket-pair Coulomb blocks are added with atomics.  Returns -1 with an
+ * error message when the build needs more scratch than the buffer holds
+ * (about nao <= 2800 for an s/p/d basis without density screening).  Every
+ * call on a GPU shares that GPU's buffer, so calls that overlap on one GPU
+ * (several streams or threads) race; calls in sequence on one stream are
+ * safe.  The context must be set up (librint_ctx_create and its set_*
+ * calls) beforehand: those allocate. */
+int librint_fock_device_noalloc(librint_ctx *ctx, const double *P_dev, double *G_dev, void *stream);
+
 /* Blocks (ij|kl) of nq shell quartets (4 ints each, indices into bas), each
  * in cint2e_cart layout (i fastest), concatenated into out.  Returns the
  * number of doubles written, or (size_t)-1 on failure.  With out == NULL only

@@ -56,6 +56,8 @@ library.librint_fock.argtypes = (ctypes.c_void_p, _dp, _dp)
 library.librint_fock.restype = ctypes.c_int
 library.librint_fock_device.argtypes = (ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
 library.librint_fock_device.restype = ctypes.c_int
+library.librint_fock_device_noalloc.argtypes = (ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+library.librint_fock_device_noalloc.restype = ctypes.c_int
 library.librint_int2e_shells.argtypes = (ctypes.c_void_p, _ip, ctypes.c_int, _dp)
 library.librint_int2e_shells.restype = _sz
 library.librint_schwarz.argtypes = (ctypes.c_void_p, _dp)
@@ -103,5 +105,5 @@ EXPORTED = (
     "librint_ctx_class_stats", "librint_ctx_set_precision", "librint_ctx_flag_counts",
     "librint_scf", "librint_last_error", "librint_ctx_gradient", "grad_nuc_c", "librint_scf_ex",
     "librint_mlp_gemm", "librint_mlp_relu_bwd", "librint_mlp_colsum", "librint_mlp_mse", "librint_mlp_adam",
-    "librint_mlp_to_bf16", "librint_ctx_set_shard",
+    "librint_mlp_to_bf16", "librint_ctx_set_shard", "librint_fock_device_noalloc",
 )

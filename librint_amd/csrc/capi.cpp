@@ -540,6 +540,59 @@ int librint_fock_device(librint_ctx *ctx, const double *P_dev, double *G_dev, vo
     }
 }
 
+}  // extern "C"
+
+// Device scratch of librint_fock_device_noalloc: a file-scope array each GPU
+// holds from the library's load until the process ends.  In host code its
+// name is only a stand-in: the device address comes from
+// hipGetSymbolAddress, once per device (with that device current), and is
+// the only handle host code passes on.
+namespace {
+constexpr size_t NOALLOC_SCRATCH_DOUBLES = (size_t)8 << 20;  // 64 MiB
+constexpr int NOALLOC_MAX_DEVICES = 64;
+__device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+double *device_scratch()
+{
+    int dev = 0;
+    librint::check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) throw std::runtime_error("device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        librint::check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int librint_fock_device_noalloc(librint_ctx *ctx, const double *P_dev, double *G_dev, void *stream)
+{
+    try {
+        if (!ctx || !P_dev || !G_dev) {
+            set_err("null argument");
+            return -1;
+        }
+        const size_t need = ctx->ctx->scratch_doubles_needed();
+        if (need > NOALLOC_SCRATCH_DOUBLES) {
+            set_err("librint_fock_device_noalloc: this build needs " + std::to_string(need * sizeof(double)) +
+                    " bytes of scratch, above the " + std::to_string(NOALLOC_SCRATCH_DOUBLES * sizeof(double)) +
+                    "-byte static buffer (use librint_fock_device)");
+            return -1;
+        }
+        ctx->ctx->fock_device_scratch(P_dev, G_dev, (hipStream_t)stream, device_scratch(), NOALLOC_SCRATCH_DOUBLES);
+        return 0;
+    } catch (std::exception &e) {
+        set_err(e.what());
+        return -1;
+    }
+}
+
 size_t librint_int2e_shells(librint_ctx *ctx, const int *quartets, int nq, double *out)
 {
     try {
