@@ -30,13 +30,15 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
 HBM_PEAK = 8000.0         # GB/s
 
-# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE runs scaled by the same counters on a copy of known size
-# (profiles/jk_pmc_traffic_c16h34_631g.md); keyed by (workload, precision, kernel).
-TRAFFIC = {
-    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, double, false>"): (16877.7 * 2.0 + 125931.6) * 1024,
-    ("c16h34_631g", "fp64", "jk_group_kernel<0, 0, 0, 0, double, false>"): (9216.0 * 2.0 + 75920.5) * 1024,
-}
+# Counter-measured figures of the dominant kernel per launch, from the
+# committed rocprofv3 summaries (profiles/valu_counters_fock.md,
+# profiles/jk_pmc_traffic_fock.md), keyed by (workload, precision, kernel):
+#   TRAFFIC: HBM bytes, FETCH_SIZE x 2 + WRITE_SIZE (KiB, each scaled by what
+#            the counter reads for a copy of known size);
+#   COUNTER_FLOPS: executed FP64 VALU flops, 64 x (2 FMA + ADD + MUL + TRANS)
+#            wavefront instructions.
+TRAFFIC = {}
+COUNTER_FLOPS = {}
 
 
 def core_guess_density(m):
@@ -297,6 +299,10 @@ def fock_line(name, steps, warmup, tau_mix_list, traffic_precision="fp64"):
                      "frac": dom_achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC.get((name, traffic_precision, dom["kernel"])),
                      "kernel": dom["kernel"], "kernel_ms": dom["ms"], "flops_per_launch": dom["flops"],
+                     "counter_flops_per_launch": COUNTER_FLOPS.get((name, traffic_precision, dom["kernel"])),
+                     "frac_counter": (COUNTER_FLOPS[(name, traffic_precision, dom["kernel"])] / (dom["ms"] * 1e-3)
+                                      / 1e12 / FP64_PEAK_TFLOPS
+                                      if (name, traffic_precision, dom["kernel"]) in COUNTER_FLOPS else None),
                      "all_classes": {"achieved": achieved_all, "frac": achieved_all / FP64_PEAK_TFLOPS,
                                      "ms": jk_ms, "flops": st["flops"]}},
         "classes": classes,

@@ -82,10 +82,12 @@ double *denergy_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, s
 
 /* --------------------------------------------------------- extensions ---- */
 
-/* Stateless direct two-electron Fock matrix G = J - K/2 (nao*nao, release
+/* Stateless direct two-electron Fock matrix G = J - K/2 (n*n, release
  * with free_c), the GPU counterpart of integral2e_fock (reference
- * src/scf.rs:316-410).  schwarz_tol: quartets with Q_ij*Q_kl <= tol are
- * skipped (0 keeps every quartet).  precision: 0 = FP64. */
+ * src/scf.rs:316-410).  coord 0: cartesian, P and G nao*nao; coord 1: real
+ * spherical, P and G nsph*nsph (the cartesian build between the spherical
+ * transforms, G_sph = C G_cart[C^T P C] C^T).  schwarz_tol: quartets with
+ * Q_ij*Q_kl <= tol are skipped (0 keeps every quartet).  precision: 0 = FP64. */
 double *fock_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l,
                int coord, int precision, double schwarz_tol);
 

@@ -99,7 +99,7 @@ double *int1e_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, siz
             set_err("invalid typec code " + std::to_string(typec));
             return nullptr;
         }
-        Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env, env_l);
         const size_t nn = (size_t)in.nao * in.nao;
         if (coord == 1) {
             DevMem d(nn);
@@ -140,7 +140,7 @@ static ScfOut run_ref_scf(int *atm, size_t atm_l, int *bas, size_t bas_l, double
 {
     Context ctx(atm, atm_l, bas, bas_l, env, env_l);
     ctx.set_schwarz(0.0);
-    Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env);
+    Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env, env_l);
     ScfOptions o;
     return scf_run(ctx, in, (int)nelec, imax, conv, o);
 }
@@ -173,7 +173,7 @@ double energy_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, siz
             set_err("P has the wrong length");
             return NAN;
         }
-        Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        Int1eInput in = make_int1e_input(atm, atm_l, bas, bas_l, env, env_l);
         std::vector<double> H(n * n), V(n * n);
         int1e_gpu(in, nullptr, H.data(), V.data());
         for (size_t i = 0; i < n * n; i++) H[i] += V[i];
@@ -281,7 +281,7 @@ double *grad_entry(const char *name, int *atm, size_t atm_l, int *bas, size_t ba
             set_err(std::string(name) + ": null argument");
             return nullptr;
         }
-        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env, env_l);
         const size_t n = (size_t)one.nao;
         if (P_l != n * n) {
             set_err(std::string(name) + ": P has length " + std::to_string(P_l) + ", expected " + std::to_string(n * n));
@@ -326,7 +326,7 @@ double *dS_u(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t
             set_err("dS_u: null argument");
             return nullptr;
         }
-        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env, env_l);
         size_t s1, s2;
         ref_split(bas, bas_l, s1, s2);
         if (s2 < s1 || s2 > env_l) {
@@ -382,7 +382,7 @@ double *grad_nuc_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, 
             set_err("grad_nuc_c: null argument");
             return nullptr;
         }
-        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        Int1eInput one = make_int1e_input(atm, atm_l, bas, bas_l, env, env_l);
         const size_t n = (size_t)one.nao;
         if (P_l != n * n) {
             set_err("grad_nuc_c: P has the wrong length");
@@ -413,12 +413,53 @@ double *fock_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size
         Context ctx(atm, atm_l, bas, bas_l, env, env_l);
         ctx.set_schwarz(schwarz_tol);
         const size_t n = (size_t)ctx.nao();
-        if (P_l != n * n) {
-            set_err("P has the wrong length");
+        if (coord == 0) {
+            if (P_l != n * n) {
+                set_err("P has the wrong length (nao*nao cartesian)");
+                return nullptr;
+            }
+            double *G = alloc_out(n * n);
+            ctx.fock_host(P, G);
+            return G;
+        }
+        // spherical (reference integral2e_fock with coord = 1): with C the
+        // sph x cart rows, the spherical integrals are C x C x C x C of the
+        // cartesian ones, so G_sph = C G_cart[C^T P C] C^T
+        int ncart = 0;
+        const std::vector<C2SRow> rows = c2s_rows(bas, bas_l, &ncart);
+        const size_t ns = rows.size();
+        if ((size_t)ncart != n || P_l != ns * ns) {
+            set_err("P has the wrong length (nsph*nsph spherical)");
             return nullptr;
         }
-        double *G = alloc_out(n * n);
-        ctx.fock_host(P, G);
+        std::vector<double> T(ns * n, 0.0), Pc(n * n, 0.0), Gc(n * n);
+        for (size_t mu = 0; mu < ns; mu++)  // T = P C  (ns x n)
+            for (size_t nu = 0; nu < ns; nu++) {
+                const double p = P[mu * ns + nu];
+                if (p == 0.0) continue;
+                for (int t = 0; t < rows[nu].n; t++) T[mu * n + rows[nu].cart[t]] += p * rows[nu].c[t];
+            }
+        for (size_t mu = 0; mu < ns; mu++)  // Pc = C^T T
+            for (int t = 0; t < rows[mu].n; t++) {
+                const size_t a = (size_t)rows[mu].cart[t];
+                const double c = rows[mu].c[t];
+                for (size_t b = 0; b < n; b++) Pc[a * n + b] += c * T[mu * n + b];
+            }
+        ctx.fock_host(Pc.data(), Gc.data());
+        std::vector<double> U(ns * n, 0.0);  // U = C Gc  (ns x n)
+        for (size_t mu = 0; mu < ns; mu++)
+            for (int t = 0; t < rows[mu].n; t++) {
+                const size_t a = (size_t)rows[mu].cart[t];
+                const double c = rows[mu].c[t];
+                for (size_t b = 0; b < n; b++) U[mu * n + b] += c * Gc[a * n + b];
+            }
+        double *G = alloc_out(ns * ns);  // G = U C^T
+        for (size_t mu = 0; mu < ns; mu++)
+            for (size_t nu = 0; nu < ns; nu++) {
+                double v = 0.0;
+                for (int t = 0; t < rows[nu].n; t++) v += U[mu * n + rows[nu].cart[t]] * rows[nu].c[t];
+                G[mu * ns + nu] = v;
+            }
         return G;
     } catch (std::exception &e) {
         set_err(e.what());
@@ -432,7 +473,7 @@ librint_ctx *librint_ctx_create(const int *atm, size_t atm_l, const int *bas, si
     try {
         auto *c = new librint_ctx;
         c->ctx.reset(new Context(atm, atm_l, bas, bas_l, env, env_l));
-        c->one = make_int1e_input(atm, atm_l, bas, bas_l, env);
+        c->one = make_int1e_input(atm, atm_l, bas, bas_l, env, env_l);
         return c;
     } catch (std::exception &e) {
         set_err(e.what());

@@ -502,7 +502,13 @@ void Context::build_tasks()
             int la, lb;
             cls_ls(std::get<0>(kv.first), la, lb);
             const int rows = ncart(la) + ncart(lb);
-            cap = std::min(256, std::max(8, (rows * nao_ + 63) / 64));
+            // lists of cheap quartets (fewer than 4 primitive quartets each,
+            // the uncontracted water-cluster lists) take groups 4 times as
+            // large: there the per-group row staging, zeroing and flush
+            // weigh on quartets that cost little (measured on (H2O)32,
+            // one stream: 46.4 -> 43.0 ms with 256-task groups)
+            const bool cheap = (double)A.stat.prim_quartets < 4.0 * (double)A.stat.quartets;
+            cap = std::min(256, std::max(8, (rows * nao_ + 63) / 64 * (cheap ? 4 : 1)));
             cap = std::max(8, std::min(cap, (int)(A.tasks.size() / 4096)));
         }
         for (size_t t = 0; t < A.tasks.size();) {
@@ -599,77 +605,92 @@ void Context::build_tasks()
         if (launches_[k].flops > launches_[dominant_].flops) dominant_ = (int)k;
 
     // Coulomb slabs: every launch of a register class stores its ket-side
-    // Coulomb blocks per task; items = (launch, 64-ket window) with the list
-    // of the launch's tasks on that window.
+    // Coulomb blocks per task (64 NFC NFD doubles) and a reduction sums the
+    // slabs of the tasks that share a 64-ket window.  The slab memory is
+    // bounded: with S doubles of slabs in all, a build keeps them all when
+    // S <= B (one reduction after every launch), else each launch runs in
+    // chunks of consecutive groups whose slabs fit B, each chunk reduced
+    // right after it on its stream, with one B-sized buffer per stream.
+    // B = LIBRINT_SLAB_MB MiB (default 1024), so the slabs take at most
+    // max(min(S, B), B x streams) doubles.
     std::vector<JRedItem> items;
     std::vector<int> item_tasks;
-    // slab budget in doubles: a launch above JBUF_LAUNCH keeps the global
-    // atomics; one reduction after the whole build while the total stays
-    // under JBUF_WHOLE, else the buffer holds one launch at a time
-    const size_t JBUF_LAUNCH = (size_t)1 << 30;  // 8 GiB
-    const size_t JBUF_WHOLE = (size_t)1 << 28;   // 2 GiB
-    const size_t JBUF_STREAMS = (size_t)1 << 31; // 16 GiB: per-stream buffers of the per-launch mode
-    size_t total = 0, largest = 0;
+    size_t budget = (size_t)1 << 27;  // 1 GiB of doubles
+    if (const char *e = getenv("LIBRINT_SLAB_MB")) budget = std::max<size_t>(1, (size_t)atoll(e)) << 17;
+    size_t total = 0;
     max_nfcd_ = 1;
     for (ClassLaunch &L : launches_) {
         L.jbase = -1;
-        L.ifirst = L.icount = 0;
         if (no_jslab_ || !class_has_fp32(L.la, L.lb, L.lc, L.ld)) continue;
-        const size_t sz = (size_t)L.count * 64 * ncart(L.lc) * ncart(L.ld);
-        if (sz > JBUF_LAUNCH) continue;
         L.jbase = (long long)total;
-        total += sz;
-        largest = std::max(largest, sz);
+        total += (size_t)L.count * 64 * ncart(L.lc) * ncart(L.ld);
     }
-    jslab_whole_ = total <= JBUF_WHOLE;
+    jslab_whole_ = total <= budget;
+    chunks_.clear();
     for (ClassLaunch &L : launches_) {
-        if (L.jbase < 0) continue;
-        if (!jslab_whole_) L.jbase = 0;
-        std::map<int, std::vector<int>> win;
-        for (int t = L.first; t < L.first + L.count; t++) win[tasks[t].ket0].push_back(t);
-        L.ifirst = (int)items.size();
-        // a window's task list is split into items of at most JRED_TASKS
-        // tasks, so that the reduction has enough workgroups
-        const int JRED_TASKS = 256;
-        for (auto &kv : win) {
-            const int cnt = (int)kv.second.size();
-            item_tasks.insert(item_tasks.end(), kv.second.begin(), kv.second.end());
-            for (int c0 = 0; c0 < cnt; c0 += JRED_TASKS) {
-                JRedItem it;
-                it.jbase = L.jbase;
-                it.tfirst = L.first;
-                it.ket0 = kv.first;
-                it.nfc = ncart(L.lc);
-                it.nfd = ncart(L.ld);
-                it.lfirst = (int)item_tasks.size() - cnt + c0;
-                it.lcount = std::min(JRED_TASKS, cnt - c0);
-                items.push_back(it);
-                max_nfcd_ = std::max(max_nfcd_, it.nfc * it.nfd);
-            }
+        L.cfirst = (int)chunks_.size();
+        const size_t per_task = 64 * (size_t)ncart(L.lc) * ncart(L.ld);
+        if (L.jbase < 0) {
+            chunks_.push_back(SlabChunk{L.gfirst, L.gcount, L.first, L.count, 0, 0, -1});
+            L.ccount = 1;
+            continue;
         }
-        L.icount = (int)items.size() - L.ifirst;
+        // consecutive groups (their tasks are consecutive) up to the budget
+        for (int g = L.gfirst; g < L.gfirst + L.gcount;) {
+            SlabChunk C{g, 0, groups[g].first, 0, (int)items.size(), 0, 0};
+            size_t sz = 0;
+            while (g < L.gfirst + L.gcount &&
+                   (C.gcount == 0 || jslab_whole_ || sz + (size_t)groups[g].count * per_task <= budget)) {
+                sz += (size_t)groups[g].count * per_task;
+                C.tcount += groups[g].count;
+                C.gcount++;
+                g++;
+            }
+            C.jbase = jslab_whole_ ? L.jbase + (long long)(C.tfirst - L.first) * (long long)per_task : 0;
+            std::map<int, std::vector<int>> win;
+            for (int t = C.tfirst; t < C.tfirst + C.tcount; t++) win[tasks[t].ket0].push_back(t);
+            // a window's task list is split into items of at most JRED_TASKS
+            // tasks, so that the reduction has enough workgroups
+            const int JRED_TASKS = 256;
+            for (auto &kv : win) {
+                const int cnt = (int)kv.second.size();
+                item_tasks.insert(item_tasks.end(), kv.second.begin(), kv.second.end());
+                for (int c0 = 0; c0 < cnt; c0 += JRED_TASKS) {
+                    JRedItem it;
+                    it.jbase = C.jbase;
+                    it.tfirst = C.tfirst;
+                    it.ket0 = kv.first;
+                    it.nfc = ncart(L.lc);
+                    it.nfd = ncart(L.ld);
+                    it.lfirst = (int)item_tasks.size() - cnt + c0;
+                    it.lcount = std::min(JRED_TASKS, cnt - c0);
+                    items.push_back(it);
+                    max_nfcd_ = std::max(max_nfcd_, it.nfc * it.nfd);
+                }
+            }
+            C.icount = (int)items.size() - C.ifirst;
+            chunks_.push_back(C);
+        }
+        L.ccount = (int)chunks_.size() - L.cfirst;
     }
     nitems_ = (int)items.size();
     dev_free(d_items_);
     dev_free(d_item_tasks_);
     d_items_ = dev_upload(items);
     d_item_tasks_ = dev_upload(item_tasks);
-    // per-launch mode: one buffer per stream, so that launches on different
+    // chunked mode: one buffer per stream, so that launches on different
     // streams never share slab memory
-    slab_streams_ = 1;
-    if (!jslab_whole_ && largest > 0)
-        slab_streams_ = (int)std::max<size_t>(
-            1, std::min<size_t>((size_t)(nstreams_ > 0 ? nstreams_ : 4), JBUF_STREAMS / largest));
-    jslot_ = largest;
-    const size_t need = jslab_whole_ ? total : largest * slab_streams_;
+    slab_streams_ = jslab_whole_ ? 1 : std::max(1, nstreams_ > 0 ? nstreams_ : 2);
+    jslot_ = jslab_whole_ ? total : budget;
+    const size_t need = jslab_whole_ ? total : budget * slab_streams_;
     if (need > nJ_) {
         dev_free(d_J_);
         check(hipMalloc(&d_J_, sizeof(double) * need), "hipMalloc");
         nJ_ = need;
     }
     if (getenv("LIBRINT_DEBUG"))
-        fprintf(stderr, "librint: coulomb slabs %zu doubles (%s), %d reduction items\n", need,
-                jslab_whole_ ? "one reduction" : "per launch", nitems_);
+        fprintf(stderr, "librint: coulomb slabs %zu of %zu doubles (%s), %zu chunks, %d reduction items\n", need, total,
+                jslab_whole_ ? "one reduction" : "chunked", chunks_.size(), nitems_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
     evc_.assign(launches_.size(), nullptr);
@@ -692,8 +713,8 @@ void Context::build_tasks()
     h_tasks_ = tasks;
     classified_ = false;
     dev_free(d_counters_);
-    check(hipMalloc(&d_counters_, sizeof(int) * 2 * std::max<size_t>(launches_.size(), 1)), "hipMalloc");
-    check(hipMemset(d_counters_, 0, sizeof(int) * 2 * std::max<size_t>(launches_.size(), 1)), "hipMemset");
+    check(hipMalloc(&d_counters_, sizeof(int) * std::max<size_t>(chunks_.size(), 1)), "hipMalloc");
+    check(hipMemset(d_counters_, 0, sizeof(int) * std::max<size_t>(chunks_.size(), 1)), "hipMemset");
 }
 
 void Context::set_precision(int mode, double tau_mix, double tau_skip)
@@ -792,7 +813,7 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
         check(hipMemsetAsync(d_M_, 0, sizeof(double) * nn, st), "hipMemsetAsync");
     }
     const DevBasis B = devb();
-    check(launch_zero_counters(counters, 2 * (int)launches_.size(), st), "zero counters");
+    check(launch_zero_counters(counters, sc ? (int)launches_.size() : (int)chunks_.size(), st), "zero counters");
     if (max_lds_ == 0) {
         int dev = 0;
         check(hipGetDevice(&dev), "hipGetDevice");
@@ -870,7 +891,6 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
         const hipStream_t sk = slot == 0 ? st : side_[slot - 1];
         double *jbuf = jslab_whole_ ? d_J_ : d_J_ + (size_t)slot * jslot_;
         check(hipEventRecord(evs_[k], sk), "hipEventRecord");
-        int *ctr = counters + 2 * k;
         const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, false);
         const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
         // Staging the bra's density rows in LDS (DROW) doubles the row space
@@ -889,25 +909,35 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
         L.drow = drow ? 1 : 0;
         const bool hl = hl_class(L.la, L.lb, L.lc, L.ld);
         L.variant = hl ? 3 : (grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2));
-        double *js = (slabs && L.jbase >= 0) ? jbuf + L.jbase : nullptr;
         if (hl) {
             const HLShape S = hl_shape(L.la, L.lb, L.lc, L.ld);
             check(launch_hl_jk(S, d_tasks_ + L.first, L.count, flags ? flags + L.first : nullptr, 3, B, D_dev, M, nao_,
                                sc ? sc->hl : hl_scratch(S), sc ? sc->hl_len : nhlscr_, sk),
                   "jk launch (l > 2)");
-        } else if (grouped)
-            check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + L.gfirst, L.gcount, d_tasks_, ctr, flags,
-                                  gmask ? gmask + L.gfirst : nullptr, 3, B, D_dev, M, nao_, js, L.first, drow,
-                                  mx, sk),
-                  "jk group launch");
-        else
-            check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + L.first, L.count, ctr,
-                            flags ? flags + L.first : nullptr, 3, B, D_dev, M, nao_, js, 0, sk),
-                  "jk launch");
-        if (slabs && !jslab_whole_ && L.icount)
-            check(launch_jreduce(d_items_ + L.ifirst, L.icount, max_nfcd_, d_item_tasks_, d_tasks_, flags, 3, jbuf, B,
-                                 M, nao_, sk),
-                  "coulomb reduction launch");
+        } else {
+            // the launch in its slab chunks (one when the slabs are kept
+            // whole or absent); with caller scratch, no slabs: one launch
+            // over every group with the ket Coulomb blocks added by atomics
+            const int c_begin = sc ? 0 : L.cfirst, c_end = sc ? 1 : L.cfirst + L.ccount;
+            for (int ci = c_begin; ci < c_end; ci++) {
+                const SlabChunk C = sc ? SlabChunk{L.gfirst, L.gcount, L.first, L.count, 0, 0, -1} : chunks_[ci];
+                int *ctr = sc ? counters + k : counters + ci;
+                double *js = (slabs && C.jbase >= 0) ? jbuf + C.jbase : nullptr;
+                if (grouped)
+                    check(launch_jk_group(L.la, L.lb, L.lc, L.ld, L.fp32, d_groups_ + C.gfirst, C.gcount, d_tasks_,
+                                          ctr, flags, gmask ? gmask + C.gfirst : nullptr, 3, B, D_dev, M, nao_, js,
+                                          C.tfirst, drow, mx, sk),
+                          "jk group launch");
+                else
+                    check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + C.tfirst, C.tcount, ctr,
+                                    flags ? flags + C.tfirst : nullptr, 3, B, D_dev, M, nao_, js, 0, sk),
+                          "jk launch");
+                if (js && !jslab_whole_ && C.icount)
+                    check(launch_jreduce(d_items_ + C.ifirst, C.icount, ncart(L.lc) * ncart(L.ld), d_item_tasks_,
+                                         d_tasks_, flags, 3, jbuf, B, M, nao_, sk),
+                          "coulomb reduction launch");
+            }
+        }
         check(hipEventRecord(evc_[k], sk), "hipEventRecord");
     }
     for (int i = 0; i < ns - 1; i++) {

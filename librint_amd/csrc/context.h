@@ -43,8 +43,8 @@ struct ClassLaunch {
     double flops = 0;
     float ms = 0.f;    // device time of this launch in the last build
     int variant = -1;  // kernel of the last build: 0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel
-    long long jbase = -1;  // Coulomb slab offset (-1: ket Coulomb block by global atomics)
-    int ifirst = 0, icount = 0;  // slab reduction items of this launch
+    long long jbase = -1;  // Coulomb slab offset in the whole-build layout (-1: global atomics)
+    int cfirst = 0, ccount = 0;  // range in the slab chunks (one chunk when the slabs are kept whole)
     int drow = 0;                // bra density rows staged in LDS in the last build
 };
 
@@ -150,6 +150,13 @@ private:
     std::vector<Bin> bins_;
 
     std::vector<ClassLaunch> launches_;
+    // consecutive groups of one launch whose Coulomb slabs fit the slab
+    // budget, with their reduction items; jbase -1: no slabs (atomics)
+    struct SlabChunk {
+        int gfirst, gcount, tfirst, tcount, ifirst, icount;
+        long long jbase;
+    };
+    std::vector<SlabChunk> chunks_;
     Stats stats_;
 
     // device

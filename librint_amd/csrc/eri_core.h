@@ -232,12 +232,14 @@ __device__ __forceinline__ void rys_nodes_lds(T x, const T *rt, T *v, T *w)
 {
     using R = RysTab<T, N, WONLY>;
     if (x >= (T)RYS_NSEG[N]) {
-        const T rs = rsqrt_t(x);
-        const T rx = rs * rs;
+        // large x: the Gauss-Hermite limit, evaluated in FP64 in both
+        // working precisions (FP32 keeps only the rounded nodes and weights)
+        const double rs = rsqrt_pos((double)x);
+        const double rx = rs * rs;
 #pragma unroll
         for (int r = 0; r < N; r++) {
-            v[r] = WONLY ? (T)0 : (T)RYS_HERM_NODE2[RYS_HOFF[N] + r] * rx;
-            w[r] = (T)RYS_HERM_W[RYS_HOFF[N] + r] * rs;
+            v[r] = WONLY ? (T)0 : (T)(RYS_HERM_NODE2[RYS_HOFF[N] + r] * rx);
+            w[r] = (T)(RYS_HERM_W[RYS_HOFF[N] + r] * rs);
         }
         return;
     }

@@ -10,7 +10,7 @@
 
 namespace librint {
 
-Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env)
+Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env, size_t env_l0)
 {
     Int1eInput in;
     in.env.assign(env, env + 0);
@@ -37,7 +37,18 @@ Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t
     for (size_t a = 0; a < atm_l / 6; a++) {
         const int pr = atm[6 * a + 1];
         env_need = std::max(env_need, (size_t)pr + 3);
+        // point nuclei only: the Gaussian charge model (nuc_mod 2 with a
+        // zeta > 0, reference src/g1e.rs:226-239) and the fractional-charge
+        // model (nuc_mod 3) are refused rather than treated as point charges
+        const int mod = atm[6 * a + 2];
+        if (mod == 3) throw std::invalid_argument("fractional nuclear charges (nuc_mod 3) are not supported");
+        if (mod == 2) {
+            const int pz = atm[6 * a + 3];
+            if (pz >= 0 && (size_t)pz < env_l0 && env[pz] > 0.0)
+                throw std::invalid_argument("Gaussian nuclear charge distributions (nuc_mod 2, zeta > 0) are not supported");
+        }
         in.charges.push_back(std::fabs((double)atm[6 * a]));
+        in.zsigned.push_back((double)atm[6 * a]);
         in.centers.push_back({env[pr], env[pr + 1], env[pr + 2]});
     }
     in.env.assign(env, env + env_need);
@@ -56,19 +67,19 @@ void nuclear_repulsion_grad(const Int1eInput &in, double *gatom)
                 d[k] = in.centers[i][k] - in.centers[j][k];
                 r2 += d[k] * d[k];
             }
-            const double f = in.charges[i] * in.charges[j] / (r2 * std::sqrt(r2));
+            const double f = in.zsigned[i] * in.zsigned[j] / (r2 * std::sqrt(r2));
             for (int k = 0; k < 3; k++) gatom[3 * i + k] -= f * d[k];
         }
 }
 
 double nuclear_repulsion(const Int1eInput &in)
-{ // reference src/scf.rs:946-953 (point charges)
+{ // reference src/scf.rs:946-953 (point charges, signed Z as given)
     double E = 0;
     for (size_t i = 0; i < in.charges.size(); i++)
         for (size_t j = 0; j < i; j++) {
             double r2 = 0;
             for (int d = 0; d < 3; d++) r2 += (in.centers[i][d] - in.centers[j][d]) * (in.centers[i][d] - in.centers[j][d]);
-            E += in.charges[i] * in.charges[j] / std::sqrt(r2);
+            E += in.zsigned[i] * in.zsigned[j] / std::sqrt(r2);
         }
     return E;
 }

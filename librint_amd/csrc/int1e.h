@@ -14,12 +14,13 @@ struct Shell1 {
 struct Int1eInput {
     std::vector<Shell1> shells;
     std::vector<double> env;
-    std::vector<double> charges;
+    std::vector<double> charges;  // |Z|: the nuclear attraction (reference src/g1e.rs:270-274)
+    std::vector<double> zsigned;  // Z as given: the nuclear repulsion (reference src/scf.rs:946-953)
     std::vector<std::array<double, 3>> centers;
     int nao = 0;
 };
 
-Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env);
+Int1eInput make_int1e_input(const int *atm, size_t atm_l, const int *bas, size_t bas_l, const double *env, size_t env_l0);
 double nuclear_repulsion(const Int1eInput &in);
 
 // On the GPU (int1e_gpu.cpp): derivatives of E1 = sum_ab W_ab O_ab (W

@@ -1184,6 +1184,8 @@ hipError_t launch_list(int la, int lb, int lc, int ld, const ListTask *tasks, in
 // in the window; the wavefronts split the item's tasks, each summing the
 // slab entries of the executed tasks whose ket range covers the lane; the
 // partial sums are combined in LDS and added to M[c][d] with one atomic.
+// Each wavefront issues the slab loads of 8 tasks before adding them, so the
+// reduction runs on memory bandwidth, not on one load latency per task.
 __global__ __launch_bounds__(256) void jreduce_kernel(const JRedItem *__restrict__ items, int nitem,
                                                       const int *__restrict__ item_tasks,
                                                       const JKTask *__restrict__ tasks,
@@ -1202,6 +1204,8 @@ __global__ __launch_bounds__(256) void jreduce_kernel(const JRedItem *__restrict
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double acc = 0.0;
     int any = 0;
+    const double *Je = J + it.jbase + e * 64 + lane;
+    const long long stride = 64LL * nfcd;
     // wave w takes the item's tasks k = w, w + 4, ...: their indices, flags
     // and ket counts are loaded at once (lane j holds task w + 4 j of the
     // current block of 256), so the slab loads below do not wait on them
@@ -1213,11 +1217,24 @@ __global__ __launch_bounds__(256) void jreduce_kernel(const JRedItem *__restrict
             if (!flags || ((accept >> flags[tl]) & 1)) nkl = tasks[tl].nket;
         }
         const int cnt = min(64, (it.lcount - base + 3) / 4);
-        for (int j = 0; j < cnt; j++) {
+        int j = 0;
+        for (; j + 8 <= cnt; j += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int t = __builtin_amdgcn_readlane(tl, j + u);
+                const int nket = __builtin_amdgcn_readlane(nkl, j + u);
+                v[u] = lane < nket ? Je[(long long)(t - it.tfirst) * stride] : 0.0;
+                any |= lane < nket;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) acc += v[u];
+        }
+        for (; j < cnt; j++) {
             const int t = __builtin_amdgcn_readlane(tl, j);
             const int nket = __builtin_amdgcn_readlane(nkl, j);
             if (lane < nket) {
-                acc += J[it.jbase + (long long)(t - it.tfirst) * (64 * nfcd) + e * 64 + lane];
+                acc += Je[(long long)(t - it.tfirst) * stride];
                 any = 1;
             }
         }

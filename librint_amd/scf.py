@@ -68,13 +68,22 @@ def scf(mol, imax: int = 200, conv: float = 1e-6) -> float:
     return E
 
 
-def fock(mol, P: np.ndarray, schwarz_tol: float = 0.0) -> np.ndarray:
-    """Stateless G = J - K/2 (the GPU integral2e_fock, reference src/scf.rs:316-410)."""
+def _coord_code(coord):
+    c = {'cart': 0, 'sph': 1}.get(coord)
+    if c is None:
+        raise ValueError("coordinate type does not exist: cart, sph")
+    return c
+
+
+def fock(mol, P: np.ndarray, schwarz_tol: float = 0.0, coord: str = 'cart') -> np.ndarray:
+    """Stateless G = J - K/2 (the GPU integral2e_fock, reference src/scf.rs:316-410);
+    coord 'cart' or 'sph' (P and G in that basis)."""
     atm, bas, env, nelec = utils.prep(mol)
-    n = utils.angl(bas, 0)
+    c = _coord_code(coord)
+    n = utils.angl(bas, c)
     P = np.ascontiguousarray(P, dtype=np.float64)
     G = library.fock_c(*utils.mol_args(atm, bas, env), P.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                       P.size, 0, 0, schwarz_tol)
+                       P.size, c, 0, schwarz_tol)
     if not G:
         raise RuntimeError("librint fock failed: " + utils.last_error())
     return utils.take(G, (n, n))

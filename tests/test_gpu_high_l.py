@@ -141,3 +141,21 @@ def test_scf_f_g_vs_oracle(gpu, mfg):
     E = scf.scf(mfg, imax=200, conv=1e-8)
     Er = refcpu.scf(mfg, imax=200, conv=1e-8)
     assert abs(E - Er) < 1e-10, (E, Er)
+
+
+def test_fock_spherical_f_g(gpu, mfg):
+    """fock_c with coord = 1 (reference integral2e_fock on the spherical
+    basis) equals J - K/2 contracted from the spherical tensor int2e_c
+    (coord = 1), which is pinned above."""
+    from librint_amd import scf
+    _say("spherical fock f/g")
+    R = scf.int2e(mfg, 'sph')
+    n = R.shape[0]
+    rng = np.random.default_rng(21)
+    P = rng.normal(size=(n, n))
+    P = P + P.T
+    G = scf.fock(mfg, P, 0.0, 'sph')
+    Gr = np.einsum("ijkl,lk->ij", R, P) - 0.5 * np.einsum("ijkl,jk->il", R, P)
+    assert G.shape == (n, n)
+    assert np.abs(G - Gr).max() < 1e-11 * np.abs(Gr).max()
+

@@ -364,6 +364,33 @@ def test_density_rows_in_lds_vs_oracle(gpu, monkeypatch):
     assert np.abs(out["0"] - Gr).max() < 1e-13 * scale
 
 
+def test_slab_budget_chunks_vs_oracle(gpu, monkeypatch):
+    """A 1 MiB Coulomb-slab budget (LIBRINT_SLAB_MB) splits every slab class
+    launch into many chunks, each reduced after it: the build still equals
+    the oracle's screened build on the same quartets, in FP64 and with the
+    mixed kernels."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("slab budget 1 MiB, (H2O)4 even-tempered")
+    m = mol.water_cluster(4)
+    P = _rand_sym(m.nao(), 17)
+    Gr, cnt = refcpu.fock2e_screened(m, P, 1e-12)
+    scale = np.abs(Gr).max()
+    out = {}
+    for mb in ("1", "1024"):
+        monkeypatch.setenv("LIBRINT_SLAB_MB", mb)
+        ctx = FockContext(m, schwarz_tol=1e-12)
+        out[mb] = ctx.fock(P)
+        assert ctx.stats()["quartets"] == cnt
+        ctx.set_precision(1, 1e-7)
+        out[mb + "m"] = ctx.fock(P)
+        ctx.close()
+    assert np.abs(out["1"] - Gr).max() < 1e-13 * scale
+    assert np.abs(out["1024"] - Gr).max() < 1e-13 * scale
+    assert np.abs(out["1m"] - out["1024m"]).max() < 1e-13 * scale
+    assert np.abs(out["1m"] - Gr).max() < 1e-6
+
+
 def test_mixed_precision_d_shells(gpu):
     from librint_amd import mol
     from librint_amd.context import FockContext

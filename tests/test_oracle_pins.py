@@ -220,3 +220,23 @@ def test_int1e_vs_textbook_general_contraction(typ):
         for j in range(m.nbas):
             blk = tb.int1e_block(m, i, j, typ)
             assert np.abs(blk - R[off[i]:off[i + 1], off[j]:off[j + 1]]).max() < 1e-13, (typ, i, j)
+
+
+def _charged_mol():
+    """H2O with a third, negative point charge carrying an H basis: the
+    nuclear repulsion uses the signed charges (reference src/scf.rs:946-953),
+    the attraction |Z| (reference src/g1e.rs:270-274)."""
+    from librint_amd import mol
+    atoms = [(8, (0.0, 0.0, 0.0)), (1, (1.43, 1.11, 0.0)), (-1, (-1.6, 0.9, 0.7))]
+    return mol.build(atoms, {8: mol.STO3G[8], 1: mol.STO3G[1], -1: mol.STO3G[1]}, "h2o_neg", normalized=True)
+
+
+def test_nuclear_repulsion_signed_charges():
+    m = _charged_mol()
+    R = m.coords()
+    Z = m.charges().astype(float)
+    ref = sum(Z[i] * Z[j] / np.linalg.norm(R[i] - R[j]) for i in range(3) for j in range(i))
+    absz = sum(abs(Z[i] * Z[j]) / np.linalg.norm(R[i] - R[j]) for i in range(3) for j in range(i))
+    assert abs(ref - absz) > 1.0  # |Z| in the repulsion would be caught
+    E = refcpu.energy(m, np.zeros((m.nao(), m.nao())))
+    assert abs(E - ref) < 1e-13

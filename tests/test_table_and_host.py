@@ -142,3 +142,25 @@ def test_library_rejects_bad_codes():
     assert not library.int1e_c(*utils.mol_args(atm, bas, env), 7, 0)
     assert not library.int1e_c(*utils.mol_args(atm, bas, env), 0, 9)
     assert not library.grad_c(*utils.mol_args(atm, bas, env), None, 0)
+
+
+@pytest.mark.gpu
+def test_library_nuclear_models(gpu):
+    """energy_c at P = 0 is the nuclear repulsion with the signed charges
+    (a negative point charge included), equal to the oracle's; a Gaussian
+    nuclear charge model (nuc_mod 2, zeta > 0) is refused, not treated as a
+    point charge."""
+    from librint_amd import scf
+    from oracle import refcpu
+    from test_oracle_pins import _charged_mol
+    m = _charged_mol()
+    n = m.nao()
+    E = scf.energy(m, np.zeros((n, n)))
+    assert abs(E - refcpu.energy(m, np.zeros((n, n)))) < 1e-13
+    # the attraction uses |Z|: V of the charged molecule equals the oracle's
+    assert np.abs(scf.int1e(m, "nuc") - refcpu.int1e(m, 2)).max() < 1e-12
+    m._atm[0][2] = 2
+    m._env[m._atm[0][3]] = 1.5
+    with pytest.raises(RuntimeError):
+        scf.int1e(m, "nuc")
+
