@@ -370,6 +370,14 @@ __host__ __device__ constexpr int GroupTabDoubles()
 // (153 VGPRs would allow 3; 128 with 5 spilled), 5 for (00|00) (102 -> 96
 // VGPRs, no spill); no request elsewhere (59 spills for 3 in (01|01)) nor
 // with the density rows in LDS (the LDS then allows two workgroups per CU).
+// Wavefronts per workgroup: 12 for (01|00) with the density rows in LDS
+// (its 73 KB of LDS at nao 992 leaves 2 workgroups of 4 per CU, 2 waves per
+// SIMD, while its registers allow 3: one workgroup of 12 holds 3), else 4.
+template <int LA, int LB, int LC, int LD, bool DROW>
+struct GroupNW {
+    static constexpr int value = (DROW && LA == 0 && LB == 1 && LC == 0 && LD == 0) ? 12 : 4;
+};
+
 template <int LA, int LB, int LC, int LD, bool DROW>
 struct GroupWaves {
     static constexpr int value = DROW ? 1
@@ -379,7 +387,8 @@ struct GroupWaves {
 };
 
 template <int LA, int LB, int LC, int LD, bool MIXED, bool DROW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(GroupWaves<LA, LB, LC, LD, DROW>::value))) void
+__global__ __launch_bounds__((64 * GroupNW<LA, LB, LC, LD, DROW>::value))
+__attribute__((amdgpu_waves_per_eu(GroupWaves<LA, LB, LC, LD, DROW>::value))) void
 jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup, const JKTask *__restrict__ tasks,
                 int *__restrict__ counter, const unsigned char *__restrict__ flags,
                 const unsigned char *__restrict__ gmask, int accept, DevBasis B, const double *__restrict__ D,
@@ -433,16 +442,17 @@ jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup, const JKTask *__
         double mab[C::NFA * C::NFB];
 #pragma unroll
         for (int e = 0; e < C::NFA * C::NFB; e++) mab[e] = 0.0;
-        // this wavefront's tasks of the group are G.first + wave + 4 j (j < 64:
-        // groups hold at most 256 tasks); lane j holds the flag of the j-th
+        // this wavefront's tasks of the group are G.first + wave + NW j (j <
+        // 64: groups hold at most 256 tasks); lane j holds the flag of the j-th
         int myflag = 0;
-        if (flags && G.first + wave + 4 * lane < G.first + G.count) myflag = flags[G.first + wave + 4 * lane];
+        constexpr int NW = GroupNW<LA, LB, LC, LD, DROW>::value;
+        if (flags && G.first + wave + NW * lane < G.first + G.count) myflag = flags[G.first + wave + NW * lane];
         double qdb = 0.0, qb = 0.0;
         if constexpr (MIXED) {
             qb = B.pair_rb[G.bra].w;
             qdb = mx.Dpair[G.bra];
         }
-        for (int t = G.first + wave, j = 0; t < G.first + G.count; t += 4, j++) {
+        for (int t = G.first + wave, j = 0; t < G.first + G.count; t += NW, j++) {
             const int fl = flags ? __builtin_amdgcn_readlane(myflag, j) : 0;
             if (!((accept >> fl) & 1)) continue;
             const JKTask tk = tasks[t];
@@ -519,9 +529,10 @@ static hipError_t launch_jk_group_d(const JKGroup *groups, int ngroup, const JKT
     const size_t lds = group_lds_bytes<LA, LB, LC, LD, MIXED>(nao, DROW);
     char what[64];
     snprintf(what, sizeof what, "jk_group<%d%d|%d%d>%s%s", LA, LB, LC, LD, MIXED ? " mixed" : "", DROW ? " drow" : "");
-    const int grid = persistent_grid((const void *)jk_group_kernel<LA, LB, LC, LD, MIXED, DROW>, 256, lds, what);
+    constexpr int threads = 64 * GroupNW<LA, LB, LC, LD, DROW>::value;
+    const int grid = persistent_grid((const void *)jk_group_kernel<LA, LB, LC, LD, MIXED, DROW>, threads, lds, what);
     const int g = std::min(grid, ngroup);
-    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, MIXED, DROW>), dim3(g), dim3(256), lds, st, groups, ngroup,
+    hipLaunchKernelGGL((jk_group_kernel<LA, LB, LC, LD, MIXED, DROW>), dim3(g), dim3(threads), lds, st, groups, ngroup,
                        tasks, counter, flags, gmask, accept, B, D, M, nao, jslab, tfirst, B.pp, mx);
     return hipGetLastError();
 }
