@@ -127,6 +127,8 @@ Context::~Context()
     dev_free(d_pair_info_);
     dev_free(d_pair_rb_);
     dev_free(d_pair_rab_);
+    dev_free(d_pair_ao_);
+    dev_free(d_pp0_);
     dev_free(d_pp_);
     dev_free(d_ao_off_);
     dev_free(d_tasks_);
@@ -166,6 +168,8 @@ DevBasis Context::devb() const
     B.pair_rab = d_pair_rab_;
     B.pp = d_pp_;
     B.ao_off = d_ao_off_;
+    B.pair_ao = d_pair_ao_;
+    B.pp0 = d_pp0_;
     return B;
 }
 
@@ -249,6 +253,16 @@ void Context::upload()
     std::vector<int> aoo(shells_.size() + 1);
     for (size_t s = 0; s < shells_.size(); s++) aoo[s] = shells_[s].ao;
     aoo[shells_.size()] = nao_;
+    std::vector<int2> pao(pairs_.size());
+    std::vector<PrimPair> pp0(pairs_.size());
+    for (size_t p = 0; p < pairs_.size(); p++) {
+        pao[p] = make_int2(shells_[pairs_[p].sa].ao, shells_[pairs_[p].sb].ao);
+        pp0[p] = pp_[pairs_[p].off];
+    }
+    dev_free(d_pair_ao_);
+    dev_free(d_pp0_);
+    d_pair_ao_ = dev_upload(pao);
+    d_pp0_ = dev_upload(pp0);
     d_pair_info_ = dev_upload(info);
     d_pair_rb_ = dev_upload(rb);
     d_pair_rab_ = dev_upload(rab);

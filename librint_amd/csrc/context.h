@@ -164,6 +164,8 @@ private:
     double4 *d_pair_rb_ = nullptr, *d_pair_rab_ = nullptr;
     PrimPair *d_pp_ = nullptr;
     int *d_ao_off_ = nullptr;
+    int2 *d_pair_ao_ = nullptr;    // AO offsets of each pair's two shells
+    PrimPair *d_pp0_ = nullptr;    // first primitive-pair record of each pair
     JKTask *d_tasks_ = nullptr;
     JKGroup *d_groups_ = nullptr;
     int *d_task_group_ = nullptr;        // group of each task

@@ -13,12 +13,18 @@ struct PrimPair;
 //   pair_rab[p]  = {Rb - Ra, 0}
 //   pp[...]      = primitive-pair records (eri_core.h)
 //   ao_off[s]    = first cartesian AO of shell s
+//   pair_ao[p]   = {ao_off[shell a], ao_off[shell b]}
+//   pp0[p]       = the first primitive-pair record of pair p (a copy of
+//                  pp[pair_info[p].z]), so that a kernel's first primitive
+//                  load does not wait on the pair's record offset
 struct DevBasis {
     const int4 *pair_info;
     const double4 *pair_rb;
     const double4 *pair_rab;
     const PrimPair *pp;
     const int *ao_off;
+    const int2 *pair_ao;
+    const PrimPair *pp0;
 };
 
 // One shell pair as the kernels read it: shells, primitive-pair records
@@ -37,8 +43,9 @@ __device__ __forceinline__ PairView load_pair(const DevBasis &B, int p)
     v.sb = pi.y;
     v.off = pi.z;
     v.npp = pi.w;
-    v.aoa = B.ao_off[v.sa];
-    v.aob = B.ao_off[v.sb];
+    const int2 ao = B.pair_ao[p];
+    v.aoa = ao.x;
+    v.aob = ao.y;
     const double4 g0 = B.pair_rb[p];
     const double4 g1 = B.pair_rab[p];
     v.rb[0] = g0.x;

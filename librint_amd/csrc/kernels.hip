@@ -95,7 +95,7 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
     // unrolled pair.
     constexpr int BUNROLL = (LA == 0 && LB == 1 && LC == 0 && LD == 0) ? 1 : 2;
     if constexpr (PIPE) {
-        PrimPair qn = B.pp[ket.off];
+        PrimPair qn = B.pp0[ketp];
         for (int ik = 0; ik < knpp; ik++) {
             const PrimPair q = qn;
             if (ik + 1 < knpp) qn = B.pp[ket.off + ik + 1];
@@ -110,7 +110,7 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
         }
     } else {
         for (int ik = 0; ik < knpp; ik++) {
-            const PrimPair q = B.pp[ket.off + ik];
+            const PrimPair q = ik == 0 ? B.pp0[ketp] : B.pp[ket.off + ik];
             const T qd[3] = {(T)q.pbx - rbd[0], (T)q.pby - rbd[1], (T)q.pbz - rbd[2]};
             for (int ib = 0; ib < bnpp; ib++) {
                 const PrimPair p = B.pp[boff + ib];
