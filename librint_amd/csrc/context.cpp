@@ -917,8 +917,12 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
         // was slower, 1.62 -> 1.65 ms).
         const size_t lds_drow = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, true);
         bool drow = drow_mode_ == 1;
+        // the rows in LDS must not cost occupancy: two 4-wave workgroups fit
+        // a CU, or the class runs 8- or 12-wave workgroups (one of them
+        // holds the 2 or 3 waves per SIMD its registers allow)
         if (drow_mode_ == 2)
-            drow = (double)L.prim_quartets < 4.0 * (double)L.quartets && lds_drow <= (size_t)max_lds_ / 2;
+            drow = (double)L.prim_quartets < 4.0 * (double)L.quartets &&
+                   (lds_drow <= (size_t)max_lds_ / 2 || jk_group_waves(L.la, L.lb, L.lc, L.ld, true) > 4);
         drow = drow && grouped && lds_drow <= (size_t)max_lds_;
         L.drow = drow ? 1 : 0;
         const bool hl = hl_class(L.la, L.lb, L.lc, L.ld);

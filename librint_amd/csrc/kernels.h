@@ -140,6 +140,9 @@ bool class_has_fp32(int la, int lb, int lc, int ld);
 // (drow: with the bra's density rows staged in LDS as well)
 // mixed: the form that picks FP32 recurrences per task (MixArgs)
 size_t jk_group_lds(int la, int lb, int lc, int ld, int mixed, int nao, bool drow);
+// wavefronts per workgroup of the grouped kernel of a class (with or without
+// the density rows in LDS)
+int jk_group_waves(int la, int lb, int lc, int ld, bool drow);
 hipError_t launch_jk_group(int la, int lb, int lc, int ld, int mixed, const JKGroup *groups, int ngroup,
                            const JKTask *tasks, int *counter, const unsigned char *flags, const unsigned char *gmask,
                            int accept, const DevBasis &B, const double *D, double *M, int nao, double *jslab,

@@ -370,12 +370,18 @@ __host__ __device__ constexpr int GroupTabDoubles()
 // (153 VGPRs would allow 3; 128 with 5 spilled), 5 for (00|00) (102 -> 96
 // VGPRs, no spill); no request elsewhere (59 spills for 3 in (01|01)) nor
 // with the density rows in LDS (the LDS then allows two workgroups per CU).
-// Wavefronts per workgroup: 12 for (01|00) with the density rows in LDS
-// (its 73 KB of LDS at nao 992 leaves 2 workgroups of 4 per CU, 2 waves per
-// SIMD, while its registers allow 3: one workgroup of 12 holds 3), else 4.
+// Wavefronts per workgroup.  Without density rows in LDS: 4.  With them:
+// 12 for (01|00) (its 73 KB of LDS at nao 992 leaves 2 workgroups of 4 per
+// CU, 2 waves per SIMD, while its registers allow 3: one workgroup of 12
+// holds 3); 8 for the classes whose registers allow 2 waves per SIMD
+// ((01|01), (11|00), (11|01), (02|00), (02|01), (12|00): one workgroup of 8
+// holds 2 even where the rows leave room for one workgroup only); 4 else.
 template <int LA, int LB, int LC, int LD, bool DROW>
 struct GroupNW {
-    static constexpr int value = (DROW && LA == 0 && LB == 1 && LC == 0 && LD == 0) ? 12 : 4;
+    static constexpr bool TWO = (LA == 0 && LB == 1 && LC == 0 && LD == 1) || (LA == 1 && LB == 1 && LC == 0 && LD == 0) ||
+                                (LA == 1 && LB == 1 && LC == 0 && LD == 1) || (LA == 0 && LB == 2 && LC == 0 && LD == 0) ||
+                                (LA == 0 && LB == 2 && LC == 0 && LD == 1) || (LA == 1 && LB == 2 && LC == 0 && LD == 0);
+    static constexpr int value = !DROW ? 4 : (LA == 0 && LB == 1 && LC == 0 && LD == 0) ? 12 : (TWO ? 8 : 4);
 };
 
 template <int LA, int LB, int LC, int LD, bool DROW>
@@ -1137,6 +1143,16 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tas
     LR_TEAM_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
+}
+
+int jk_group_waves(int la, int lb, int lc, int ld, bool drow)
+{
+#define X(a, b, c, d)                                                                                                  \
+    if (la == a && lb == b && lc == c && ld == d)                                                                      \
+        return drow ? GroupNW<a, b, c, d, true>::value : GroupNW<a, b, c, d, false>::value;
+    LR_REG_CLASSES(X)
+#undef X
+    return 4;
 }
 
 size_t jk_group_lds(int la, int lb, int lc, int ld, int mixed, int nao, bool drow)
