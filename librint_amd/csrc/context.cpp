@@ -116,6 +116,11 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     if (const char *e = getenv("LIBRINT_STREAMS")) nstreams_ = std::max(1, std::min(8, atoi(e)));
     if (const char *e = getenv("LIBRINT_TEAM_KETS")) team_kets_ = std::max(1, std::min(64, atoi(e)));
     if (const char *e = getenv("LIBRINT_PRIM_SPLIT")) prim_split_ = std::max(0, atoi(e));
+    {
+        int dev = 0;
+        check(hipGetDevice(&dev), "hipGetDevice");
+        check(hipDeviceGetAttribute(&max_lds_, hipDeviceAttributeMaxSharedMemoryPerBlock, dev), "hipDeviceGetAttribute");
+    }
     build_pairs();
     upload();
     compute_schwarz();
@@ -417,6 +422,15 @@ static double flops_prim(int la, int lb, int lc, int ld)
     return setup + roots + nr * (coef + 3 * (vrr + hrr) + gout);
 }
 
+// A chunked class runs lane per quartet (64-ket tasks with Coulomb slabs,
+// FP32 in mixed precision) only in the group kernel, i.e. where its LDS rows
+// fit; elsewhere it takes the team kernel in FP64 with atomics.
+bool Context::lane_path(int la, int lb, int lc, int ld, int mixed) const
+{
+    if (!class_split(la, lb, lc, ld)) return true;
+    return !no_group_ && jk_group_lds(la, lb, lc, ld, mixed ? 1 : 0, nao_, false) <= (size_t)max_lds_;
+}
+
 void Context::build_tasks()
 {
     // the lists below replace device buffers that launches of an earlier
@@ -485,9 +499,10 @@ void Context::build_tasks()
                     // ket, Coulomb slabs per window); the classes whose
                     // wavefront works through its kets one at a time take
                     // short windows, so that no task runs far longer than the rest
-                    const int win = class_has_fp32(la, lb, lc, ld) ? 64 : team_kets_;
+                    const bool lane = class_has_fp32(la, lb, lc, ld) && lane_path(la, lb, lc, ld, mixed);
+                    const int win = lane ? 64 : team_kets_;
                     int parts = 1;
-                    if (prim_split_ > 0 && class_has_fp32(la, lb, lc, ld) && bx.npp < 65536)
+                    if (prim_split_ > 0 && lane && bx.npp < 65536)
                         parts = std::min(bx.npp, (bx.npp * by.npp + prim_split_ - 1) / prim_split_);
                     for (int k0 = k_begin; k0 < k_end; k0 += win) {
                         const int nk = std::min(win, k_end - k0);
@@ -596,7 +611,7 @@ void Context::build_tasks()
         L.lb = lb;
         L.lc = lc;
         L.ld = ld;
-        L.fp32 = mixed && class_has_fp32(la, lb, lc, ld) ? 1 : 0;
+        L.fp32 = mixed && class_has_fp32(la, lb, lc, ld) && lane_path(la, lb, lc, ld, mixed) ? 1 : 0;
         L.first = (int)tasks.size();
         L.gfirst = (int)groups.size();
         for (auto &g : A.groups) {
@@ -635,7 +650,8 @@ void Context::build_tasks()
     max_nfcd_ = 1;
     for (ClassLaunch &L : launches_) {
         L.jbase = -1;
-        if (no_jslab_ || !class_has_fp32(L.la, L.lb, L.lc, L.ld)) continue;
+        if (no_jslab_ || !class_has_fp32(L.la, L.lb, L.lc, L.ld) || !lane_path(L.la, L.lb, L.lc, L.ld, L.fp32))
+            continue;
         L.jbase = (long long)total;
         total += (size_t)L.count * 64 * ncart(L.lc) * ncart(L.ld);
     }
@@ -926,7 +942,10 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
         drow = drow && grouped && lds_drow <= (size_t)max_lds_;
         L.drow = drow ? 1 : 0;
         const bool hl = hl_class(L.la, L.lb, L.lc, L.ld);
-        L.variant = hl ? 3 : (grouped ? 1 : (class_has_fp32(L.la, L.lb, L.lc, L.ld) ? 0 : 2));
+        L.variant = hl ? 3
+                       : (grouped ? 1
+                                  : (class_has_fp32(L.la, L.lb, L.lc, L.ld) && !class_split(L.la, L.lb, L.lc, L.ld) ? 0
+                                                                                                                 : 2));
         if (hl) {
             const HLShape S = hl_shape(L.la, L.lb, L.lc, L.ld);
             check(launch_hl_jk(S, d_tasks_ + L.first, L.count, flags ? flags + L.first : nullptr, 3, B, D_dev, M, nao_,
@@ -1216,7 +1235,7 @@ void Context::build_grad_tasks()
                 int la, lb, lc, ld;
                 cls_ls(bx.cls, la, lb);
                 cls_ls(by.cls, lc, ld);
-                const int win = class_has_fp32(la, lb, lc, ld) ? 64 : team_kets_;  // as build_tasks
+                const int win = class_has_fp32(la, lb, lc, ld) && lane_path(la, lb, lc, ld, 0) ? 64 : team_kets_;
                 for (int i0 = X == Y ? j : 0; i0 < lo; i0 += win)
                     v.push_back(JKTask{k, bx.first + i0, std::min(win, lo - i0), 0});
             }

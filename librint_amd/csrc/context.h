@@ -117,6 +117,7 @@ private:
     void build_pairs();
     void compute_schwarz();
     void build_tasks();
+    bool lane_path(int la, int lb, int lc, int ld, int mixed) const;
     void upload();
     void ensure_side_streams(int n);
     struct FockScratch {

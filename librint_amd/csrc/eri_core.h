@@ -116,24 +116,45 @@ __device__ __forceinline__ void hrr_axis(T *g, T rab, T rcd)
             }
 }
 
-// Accumulate sum_r gx*gy*gz of one root into the contracted block.
-template <int LA, int LB, int LC, int LD, typename T, typename A>
+// A chunk of the contracted block: the outputs whose component index on
+// centre SD (0 a, 1 b, 2 c, 3 d) equals S0, or the whole block for SD < 0.
+// Classes whose block does not fit a lane's registers evaluate it chunk by
+// chunk (the primitive loop repeated per chunk), so that only one chunk's
+// outputs are live at a time.  Chunk-local index
+//   f = (fa - A0) + NA ((fb - B0) + NB ((fc - C0) + NC (fd - D0))).
+template <int LA, int LB, int LC, int LD, int SD, int S0>
+struct Chunk {
+    using C = Class<LA, LB, LC, LD>;
+    static constexpr int A0 = SD == 0 ? S0 : 0, NA = SD == 0 ? 1 : C::NFA;
+    static constexpr int B0 = SD == 1 ? S0 : 0, NB = SD == 1 ? 1 : C::NFB;
+    static constexpr int C0 = SD == 2 ? S0 : 0, NC = SD == 2 ? 1 : C::NFC;
+    static constexpr int D0 = SD == 3 ? S0 : 0, ND = SD == 3 ? 1 : C::NFD;
+    static constexpr int NFL = NA * NB * NC * ND;
+    __host__ __device__ static constexpr int idx(int fa, int fb, int fc, int fd)
+    {
+        return (fa - A0) + NA * ((fb - B0) + NB * ((fc - C0) + NC * (fd - D0)));
+    }
+};
+
+// Accumulate sum_r gx*gy*gz of one root into the contracted block (or the
+// chunk (SD, S0) of it).
+template <int LA, int LB, int LC, int LD, typename T, typename A, int SD = -1, int S0 = 0>
 __device__ __forceinline__ void gout_acc(A *eri, const T *gx, const T *gy, const T *gz)
 {
     using C = Class<LA, LB, LC, LD>;
+    using K = Chunk<LA, LB, LC, LD, SD, S0>;
 #pragma unroll
-    for (int fd = 0; fd < C::NFD; fd++)
+    for (int fd = K::D0; fd < K::D0 + K::ND; fd++)
 #pragma unroll
-        for (int fc = 0; fc < C::NFC; fc++)
+        for (int fc = K::C0; fc < K::C0 + K::NC; fc++)
 #pragma unroll
-            for (int fb = 0; fb < C::NFB; fb++)
+            for (int fb = K::B0; fb < K::B0 + K::NB; fb++)
 #pragma unroll
-                for (int fa = 0; fa < C::NFA; fa++) {
+                for (int fa = K::A0; fa < K::A0 + K::NA; fa++) {
                     const int ix = cart_x(LB, fb) * C::DJ + cart_x(LD, fd) * C::DL + cart_x(LC, fc) * C::DK + cart_x(LA, fa);
                     const int iy = cart_y(LB, fb) * C::DJ + cart_y(LD, fd) * C::DL + cart_y(LC, fc) * C::DK + cart_y(LA, fa);
                     const int iz = cart_z(LB, fb) * C::DJ + cart_z(LD, fd) * C::DL + cart_z(LC, fc) * C::DK + cart_z(LA, fa);
-                    const int f = fa + C::NFA * (fb + C::NFB * (fc + C::NFC * fd));
-                    eri[f] += (A)(gx[ix] * gy[iy] * gz[iz]);
+                    eri[K::idx(fa, fb, fc, fd)] += (A)(gx[ix] * gy[iy] * gz[iz]);
                 }
 }
 
@@ -259,6 +280,25 @@ __device__ __forceinline__ void rys_nodes_lds(T x, const T *rt, T *v, T *w)
     }
 }
 
+// Node and weight of root r only (run-time r), table block in LDS.
+template <typename T, int N>
+__device__ __forceinline__ void rys_one_lds(T x, const T *rt, int r, T &v, T &w)
+{
+    using R = RysTab<T, N, false>;
+    if (x >= (T)RYS_NSEG[N]) {
+        const double rs = rsqrt_pos((double)x);
+        v = (T)(RYS_HERM_NODE2[RYS_HOFF[N] + r] * (rs * rs));
+        w = (T)(RYS_HERM_W[RYS_HOFF[N] + r] * rs);
+        return;
+    }
+    const int k = (int)x;
+    const T s = (T)2 * (x - (T)k) - (T)1;
+    const T s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
+    const T *c = rt + k * R::STRIDE;
+    v = estrin_t<T>(c + r, R::NV, s, s2, s4, s8);
+    w = estrin_t<T>(c + N + r, R::NV, s, s2, s4, s8);
+}
+
 template <int LA, int LB, int LC, int LD, typename T = double>
 struct RysLds {
     using C = Class<LA, LB, LC, LD>;
@@ -274,7 +314,7 @@ struct RysLds {
 // rcd = Rd - Rc.
 // Contribution of one Rys root (node v = t^2, weight w) of one primitive
 // quartet: recurrence coefficients, VRR, HRR and the gout products.
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, typename T, int SD = -1, int S0 = 0>
 __device__ __forceinline__ void root_term(T *eri, const PrimPair &p, const PrimPair &q, const T pq[3], T aij, T akl,
                                           T rs, T fac1, T vr, T wr, const T rab[3], const T rcd[3])
 {
@@ -297,14 +337,14 @@ __device__ __forceinline__ void root_term(T *eri, const PrimPair &p, const PrimP
         hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gy, rab[1], rcd[1]);
         hrr_axis<LA, LB, LC, C::NMAX, C::MMAX, C::DK, C::DL, C::DJ>(gz, rab[2], rcd[2]);
     }
-    gout_acc<LA, LB, LC, LD>(eri, gx, gy, gz);
+    gout_acc<LA, LB, LC, LD, T, T, SD, S0>(eri, gx, gy, gz);
 }
 
 // One primitive quartet, all roots, in working precision T with the Rys
 // table block in LDS; accumulated into eri.  qd = (Q - Rd) - (Rb - Rd) =
 // Q - Rb of the ket primitive pair (P - Q = (P - Rb) - qd), rab = Rb - Ra,
 // rcd = Rd - Rc.
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, typename T, int SD = -1, int S0 = 0>
 __device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, const PrimPair &q, const T qd[3],
                                                  const T rab[3], const T rcd[3], const T *rt)
 {
@@ -317,10 +357,23 @@ __device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, cons
     const T pq[3] = {(T)p.pbx - qd[0], (T)p.pby - qd[1], (T)p.pbz - qd[2]};
     const T rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
     const T fac1 = (T)p.K * (T)q.K * rsq;
-    T v[N], w[N];
-    rys_nodes_lds<T, N, RysLds<LA, LB, LC, LD, T>::WONLY>(a0 * rr, rt, v, w);
+    if constexpr (SD >= 0) {
+        // chunked classes: one root at a time (rolled), so that only one
+        // root's recurrence values are live
+        const T x = a0 * rr;
+#pragma unroll 1
+        for (int r = 0; r < N; r++) {
+            T vr, wr;
+            rys_one_lds<T, N>(x, rt, r, vr, wr);
+            root_term<LA, LB, LC, LD, T, SD, S0>(eri, p, q, pq, aij, akl, rs, fac1, vr, wr, rab, rcd);
+        }
+    } else {
+        T v[N], w[N];
+        rys_nodes_lds<T, N, RysLds<LA, LB, LC, LD, T>::WONLY>(a0 * rr, rt, v, w);
 #pragma unroll
-    for (int r = 0; r < N; r++) root_term<LA, LB, LC, LD, T>(eri, p, q, pq, aij, akl, rs, fac1, v[r], w[r], rab, rcd);
+        for (int r = 0; r < N; r++)
+            root_term<LA, LB, LC, LD, T, SD, S0>(eri, p, q, pq, aij, akl, rs, fac1, v[r], w[r], rab, rcd);
+    }
 }
 
 // One primitive quartet, all roots, accumulated into eri (double precision).

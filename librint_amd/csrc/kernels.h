@@ -135,6 +135,10 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tas
 hipError_t launch_zero_counters(int *p, int n, hipStream_t st);
 hipError_t launch_zero_doubles(double *p, size_t n, hipStream_t st);
 bool class_has_fp32(int la, int lb, int lc, int ld);
+// classes whose contracted block the group kernel evaluates in chunks; they
+// fall back to the team kernel (FP64, no Coulomb slabs) where the group
+// kernel's LDS rows do not fit
+bool class_split(int la, int lb, int lc, int ld);
 // grouped form (exchange rows of the bra in LDS); jk_group_lds gives its LDS
 // bytes per workgroup, (size_t)-1 for classes without a grouped kernel
 // (drow: with the bra's density rows staged in LDS as well)

@@ -20,6 +20,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <type_traits>
 
 #include "eri_core.h"
 #include "kernels.h"
@@ -66,13 +67,13 @@ __device__ __forceinline__ void contracted_fp64(double *eri, const DevBasis &B, 
 // Working precision T: the primitive loop, Rys evaluation and the
 // contracted block run in T; the J/K contraction and every accumulation
 // into M are FP64.
-template <int LA, int LB, int LC, int LD, typename T>
+template <int LA, int LB, int LC, int LD, typename T, int SD = -1, int S0 = 0>
 __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &bra, const PairView &ket, int ketp,
                                                const DevBasis &B, T *eri, const T *rt)
 {
     using C = Class<LA, LB, LC, LD>;
 #pragma unroll
-    for (int f = 0; f < C::NF; f++) eri[f] = (T)0;
+    for (int f = 0; f < Chunk<LA, LB, LC, LD, SD, S0>::NFL; f++) eri[f] = (T)0;
     const T rbd[3] = {(T)(bra.rb[0] - ket.rb[0]), (T)(bra.rb[1] - ket.rb[1]), (T)(bra.rb[2] - ket.rb[2])};
     const T rab[3] = {(T)bra.rab[0], (T)bra.rab[1], (T)bra.rab[2]};
     const T rcd[3] = {(T)ket.rab[0], (T)ket.rab[1], (T)ket.rab[2]};
@@ -105,7 +106,7 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
             for (int ib = 0; ib < bnpp; ib++) {
                 const PrimPair p = pn;
                 if (ib + 1 < bnpp) pn = B.pp[boff + ib + 1];
-                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, qd, rab, rcd, rt);
+                prim_quartet_lds<LA, LB, LC, LD, T, SD, S0>(eri, p, q, qd, rab, rcd, rt);
             }
         }
     } else {
@@ -114,7 +115,7 @@ __device__ __forceinline__ void contract_block(const JKTask &t, const PairView &
             const T qd[3] = {(T)q.pbx - rbd[0], (T)q.pby - rbd[1], (T)q.pbz - rbd[2]};
             for (int ib = 0; ib < bnpp; ib++) {
                 const PrimPair p = B.pp[boff + ib];
-                prim_quartet_lds<LA, LB, LC, LD, T>(eri, p, q, qd, rab, rcd, rt);
+                prim_quartet_lds<LA, LB, LC, LD, T, SD, S0>(eri, p, q, qd, rab, rcd, rt);
             }
         }
     }
@@ -246,16 +247,178 @@ __global__ __launch_bounds__(256) void jk_kernel(const JKTask *__restrict__ task
 // flushed once per group with coalesced global atomics; the bra-bra Coulomb
 // block is reduced over the whole group.  Only the ket-ket Coulomb block
 // (M[c][d], one atomic per element per quartet) goes straight to memory.
+// Classes whose contracted block is evaluated in chunks (DIM: the centre
+// whose component index selects a chunk, 0 a .. 3 d; N chunks): their block
+// (162-324 integrals) does not fit a lane's registers.  Each chunk repeats
+// the primitive loop (Rys nodes, VRR, HRR) and keeps only its own outputs,
+// so a lane still evaluates a whole quartet.  On the uncontracted lists
+// these classes have 1-2 primitive quartets per quartet, so the repeated
+// recurrences cost less than giving a quartet a whole wavefront.
+template <int LA, int LB, int LC, int LD>
+struct Split {
+    static constexpr int DIM = -1, N = 1;
+};
+template <> struct Split<1, 2, 1, 1> { static constexpr int DIM = 1, N = 6; };
+template <> struct Split<1, 2, 1, 2> { static constexpr int DIM = 1, N = 6; };
+template <> struct Split<2, 2, 1, 1> { static constexpr int DIM = 0, N = 6; };
+template <> struct Split<2, 2, 0, 2> { static constexpr int DIM = 0, N = 6; };
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// One task of a chunked class (lane = ket of the task): per chunk, the
+// chunk's integrals, then its share of every J/K term.  The bra Coulomb
+// entries of a chunk are reduced over the wavefront and added to the
+// group's block in LDS (mab_l); the ket Coulomb block is summed over the
+// chunks in registers; the exchange terms go to the LDS rows as in the
+// whole-block form.  Inactive lanes (past the task's kets) evaluate a valid
+// ket with weight 0 and take part in the reductions only.
+template <int LA, int LB, int LC, int LD, typename T, bool DROW>
+__device__ __forceinline__ void jk_task_chunks(const JKTask &t, const PairView &bra, const PairView &ket, int ketp,
+                                               bool active, const DevBasis &B, const double *__restrict__ D,
+                                               double *__restrict__ M, int nao, int lane, const T *rt, double *Krow,
+                                               double *jslab, const double *Drow, double *mab_l)
+{
+    using C = Class<LA, LB, LC, LD>;
+    using SP = Split<LA, LB, LC, LD>;
+    double s = active ? 1.0 : 0.0;
+    if (bra.sa == bra.sb) s *= 0.5;
+    if (ket.sa == ket.sb) s *= 0.5;
+    if (t.bra == ketp) s *= 0.5;
+    const double h = -0.5 * s;
+    const size_t n = (size_t)nao;
+    double *Ka = Krow, *Kb = Krow + C::NFA * n;
+    double jcd[C::NFC * C::NFD];
+#pragma unroll
+    for (int e = 0; e < C::NFC * C::NFD; e++) jcd[e] = 0.0;
+    // a run-time loop over the chunks (one compiled body per chunk behind a
+    // wavefront-uniform branch): unrolled, the compiler overlaps the chunks
+    // and keeps several chunks' recurrence values live
+#pragma unroll 1
+    for (int cidx = 0; cidx < SP::N; cidx++)
+    static_for<0, SP::N>([&](auto ch) {
+        constexpr int S0 = decltype(ch)::value;
+        if (cidx != S0) return;
+        using K = Chunk<LA, LB, LC, LD, SP::DIM, S0>;
+        // The density pointers pass through an empty asm statement per
+        // chunk, so that each chunk loads its density values again: reusing
+        // the first chunk's loads would keep every chunk's values live at
+        // once (81-162 doubles), far beyond the registers a lane has.
+        const double *Dk = D, *Dr = Drow;
+        asm volatile("" : "+s"(Dk));
+        if constexpr (DROW) asm volatile("" : "+s"(Dr));
+        auto Da = [&](int fa, size_t col) { return DROW ? Dr[fa * n + col] : Dk[(bra.aoa + fa) * n + col]; };
+        auto Db = [&](int fb, size_t col) {
+            return DROW ? Dr[(C::NFA + fb) * n + col] : Dk[(bra.aob + fb) * n + col];
+        };
+        T eri[K::NFL];
+        contract_block<LA, LB, LC, LD, T, SP::DIM, S0>(t, bra, ket, ketp, B, eri, rt);
+#define ERI_(fa, fb, fc, fd) ((double)eri[K::idx(fa, fb, fc, fd)])
+#pragma unroll
+        for (int fa = K::A0; fa < K::A0 + K::NA; fa++)
+#pragma unroll
+            for (int fb = K::B0; fb < K::B0 + K::NB; fb++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int fd = K::D0; fd < K::D0 + K::ND; fd++)
+#pragma unroll
+                    for (int fc = K::C0; fc < K::C0 + K::NC; fc++)
+                        acc += Dk[(ket.aoa + fc) * n + ket.aob + fd] * ERI_(fa, fb, fc, fd);
+                const double v = wave_sum(2.0 * s * acc);
+                if (lane == 0) atomicAdd(&mab_l[fa * C::NFB + fb], v);
+            }
+#pragma unroll
+        for (int fc = K::C0; fc < K::C0 + K::NC; fc++)
+#pragma unroll
+            for (int fd = K::D0; fd < K::D0 + K::ND; fd++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int fb = K::B0; fb < K::B0 + K::NB; fb++)
+#pragma unroll
+                    for (int fa = K::A0; fa < K::A0 + K::NA; fa++)
+                        acc += Dk[(bra.aoa + fa) * n + bra.aob + fb] * ERI_(fa, fb, fc, fd);
+                jcd[fc * C::NFD + fd] += 2.0 * s * acc;
+            }
+        if (active) {
+#pragma unroll
+            for (int fc = K::C0; fc < K::C0 + K::NC; fc++) {
+#pragma unroll
+                for (int fa = K::A0; fa < K::A0 + K::NA; fa++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int fd = K::D0; fd < K::D0 + K::ND; fd++)
+#pragma unroll
+                        for (int fb = K::B0; fb < K::B0 + K::NB; fb++) acc += Db(fb, ket.aob + fd) * ERI_(fa, fb, fc, fd);
+                    atomicAdd(&Ka[fa * n + ket.aoa + fc], h * acc);
+                }
+#pragma unroll
+                for (int fb = K::B0; fb < K::B0 + K::NB; fb++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int fd = K::D0; fd < K::D0 + K::ND; fd++)
+#pragma unroll
+                        for (int fa = K::A0; fa < K::A0 + K::NA; fa++) acc += Da(fa, ket.aob + fd) * ERI_(fa, fb, fc, fd);
+                    atomicAdd(&Kb[fb * n + ket.aoa + fc], h * acc);
+                }
+            }
+#pragma unroll
+            for (int fd = K::D0; fd < K::D0 + K::ND; fd++) {
+#pragma unroll
+                for (int fa = K::A0; fa < K::A0 + K::NA; fa++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int fc = K::C0; fc < K::C0 + K::NC; fc++)
+#pragma unroll
+                        for (int fb = K::B0; fb < K::B0 + K::NB; fb++) acc += Db(fb, ket.aoa + fc) * ERI_(fa, fb, fc, fd);
+                    atomicAdd(&Ka[fa * n + ket.aob + fd], h * acc);
+                }
+#pragma unroll
+                for (int fb = K::B0; fb < K::B0 + K::NB; fb++) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int fc = K::C0; fc < K::C0 + K::NC; fc++)
+#pragma unroll
+                        for (int fa = K::A0; fa < K::A0 + K::NA; fa++) acc += Da(fa, ket.aoa + fc) * ERI_(fa, fb, fc, fd);
+                    atomicAdd(&Kb[fb * n + ket.aob + fd], h * acc);
+                }
+            }
+        }
+#undef ERI_
+    });
+    if (!active) return;
+#pragma unroll
+    for (int fc = 0; fc < C::NFC; fc++)
+#pragma unroll
+        for (int fd = 0; fd < C::NFD; fd++) {
+            if (jslab)
+                jslab[(fc * C::NFD + fd) * 64 + lane] = jcd[fc * C::NFD + fd];
+            else
+                atomicAdd(&M[(ket.aoa + fc) * n + ket.aob + fd], jcd[fc * C::NFD + fd]);
+        }
+}
+
 template <int LA, int LB, int LC, int LD, typename T, bool DROW>
 __device__ __forceinline__ void jk_task_rows(const JKTask &t, const PairView &bra, const DevBasis &B,
                                              const double *__restrict__ D, double *__restrict__ M, int nao, int lane,
                                              const T *rt, double *Krow, double *mab, double *jslab,
-                                             const double *Drow)
+                                             const double *Drow, double *mab_l)
 {
     using C = Class<LA, LB, LC, LD>;
+    using SP = Split<LA, LB, LC, LD>;
     const bool active = lane < t.nket;
     const int ketp = t.ket0 + (active ? lane : 0);
     const PairView ket = load_pair(B, ketp);
+    if constexpr (SP::N > 1) {
+        jk_task_chunks<LA, LB, LC, LD, T, DROW>(t, bra, ket, ketp, active, B, D, M, nao, lane, rt, Krow, jslab, Drow,
+                                                mab_l);
+        return;
+    }
     T eri[C::NF];
     contract_block<LA, LB, LC, LD, T>(t, bra, ket, ketp, B, eri, rt);
     if (!active) return;
@@ -470,14 +633,18 @@ jk_group_kernel(const JKGroup *__restrict__ groups, int ngroup, const JKTask *__
                 if (lane == 0) mx.prec[t] = f32 ? 1 : 0;
             }
             if (f32)
-                jk_task_rows<LA, LB, LC, LD, float, DROW>(tk, bra, B, D, M, nao, lane, rt32, Krow, mab, js, Drow);
+                jk_task_rows<LA, LB, LC, LD, float, DROW>(tk, bra, B, D, M, nao, lane, rt32, Krow, mab, js, Drow,
+                                                          mab_l);
             else
-                jk_task_rows<LA, LB, LC, LD, double, DROW>(tk, bra, B, D, M, nao, lane, rt, Krow, mab, js, Drow);
+                jk_task_rows<LA, LB, LC, LD, double, DROW>(tk, bra, B, D, M, nao, lane, rt, Krow, mab, js, Drow,
+                                                           mab_l);
         }
+        if constexpr (Split<LA, LB, LC, LD>::N == 1) {
 #pragma unroll
-        for (int e = 0; e < C::NFA * C::NFB; e++) {
-            const double v = wave_sum(mab[e]);
-            if (lane == 0) atomicAdd(&mab_l[e], v);
+            for (int e = 0; e < C::NFA * C::NFB; e++) {
+                const double v = wave_sum(mab[e]);
+                if (lane == 0) atomicAdd(&mab_l[e], v);
+            }
         }
         __syncthreads();
         for (int i = threadIdx.x; i < nrow; i += blockDim.x) {
@@ -1054,20 +1221,33 @@ __global__ void symmetrize_kernel(const double *__restrict__ M, double *__restri
 #define LR_REG_CLASSES(X)                                                                                              \
     X(0, 0, 0, 0) X(0, 1, 0, 0) X(0, 1, 0, 1) X(1, 1, 0, 0) X(1, 1, 0, 1) X(1, 1, 1, 1) X(0, 2, 0, 0) X(0, 2, 0, 1)   \
         X(0, 2, 1, 1) X(0, 2, 0, 2) X(1, 2, 0, 0) X(1, 2, 0, 1) X(1, 2, 0, 2) X(2, 2, 0, 0) X(2, 2, 0, 1)
-#define LR_TEAM_CLASSES(X) X(1, 2, 1, 1) X(1, 2, 1, 2) X(2, 2, 1, 1) X(2, 2, 0, 2) X(2, 2, 1, 2) X(2, 2, 2, 2)
+// SPLIT: contracted block in registers chunk by chunk (Split<>), team
+// kernel when the group kernel's LDS rows do not fit
+#define LR_SPLIT_CLASSES(X) X(1, 2, 1, 1) X(1, 2, 1, 2) X(2, 2, 1, 1) X(2, 2, 0, 2)
+#define LR_TEAM_CLASSES(X) X(2, 2, 1, 2) X(2, 2, 2, 2)
 // the list kernel keeps fewer live values than the J/K kernel but no fewer
 // outputs; these two also go to the team form there
 #define LR_LIST_REG_CLASSES(X)                                                                                         \
     X(0, 0, 0, 0) X(0, 1, 0, 0) X(0, 1, 0, 1) X(1, 1, 0, 0) X(1, 1, 0, 1) X(1, 1, 1, 1) X(0, 2, 0, 0) X(0, 2, 0, 1)   \
         X(0, 2, 1, 1) X(0, 2, 0, 2) X(1, 2, 0, 0) X(1, 2, 0, 1) X(2, 2, 0, 0)
-#define LR_LIST_TEAM_CLASSES(X) LR_TEAM_CLASSES(X) X(1, 2, 0, 2) X(2, 2, 0, 1)
+#define LR_LIST_TEAM_CLASSES(X) LR_SPLIT_CLASSES(X) LR_TEAM_CLASSES(X) X(1, 2, 0, 2) X(2, 2, 0, 1)
 
 bool class_supported(int la, int lb, int lc, int ld)
 {
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d) return true;
     LR_REG_CLASSES(X)
+    LR_SPLIT_CLASSES(X)
     LR_TEAM_CLASSES(X)
+#undef X
+    return false;
+}
+
+bool class_split(int la, int lb, int lc, int ld)
+{
+#define X(a, b, c, d)                                                                                                  \
+    if (la == a && lb == b && lc == c && ld == d) return true;
+    LR_SPLIT_CLASSES(X)
 #undef X
     return false;
 }
@@ -1118,6 +1298,7 @@ bool class_has_fp32(int la, int lb, int lc, int ld)
 #define X(a, b, c, d)                                                                                                  \
     if (la == a && lb == b && lc == c && ld == d) return true;
     LR_REG_CLASSES(X)
+    LR_SPLIT_CLASSES(X)
 #undef X
     return false;
 }
@@ -1140,6 +1321,7 @@ hipError_t launch_jk(int la, int lb, int lc, int ld, int fp32, const JKTask *tas
                            nao);                                                                                       \
         return hipGetLastError();                                                                                      \
     }
+    LR_SPLIT_CLASSES(X)
     LR_TEAM_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
@@ -1151,6 +1333,7 @@ int jk_group_waves(int la, int lb, int lc, int ld, bool drow)
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
         return drow ? GroupNW<a, b, c, d, true>::value : GroupNW<a, b, c, d, false>::value;
     LR_REG_CLASSES(X)
+    LR_SPLIT_CLASSES(X)
 #undef X
     return 4;
 }
@@ -1161,6 +1344,7 @@ size_t jk_group_lds(int la, int lb, int lc, int ld, int mixed, int nao, bool dro
     if (la == a && lb == b && lc == c && ld == d)                                                                      \
         return mixed ? group_lds_bytes<a, b, c, d, true>(nao, drow) : group_lds_bytes<a, b, c, d, false>(nao, drow);
     LR_REG_CLASSES(X)
+    LR_SPLIT_CLASSES(X)
 #undef X
     return (size_t)-1;
 }
@@ -1179,6 +1363,7 @@ hipError_t launch_jk_group(int la, int lb, int lc, int ld, int mixed, const JKGr
                      : launch_jk_group_t<a, b, c, d, false>(groups, ngroup, tasks, counter, flags, gmask, accept, B,   \
                                                             D, M, nao, jslab, tfirst, drow, mx, st);
     LR_REG_CLASSES(X)
+    LR_SPLIT_CLASSES(X)
 #undef X
     return hipErrorInvalidValue;
 }

@@ -391,6 +391,40 @@ def test_slab_budget_chunks_vs_oracle(gpu, monkeypatch):
     assert np.abs(out["1m"] - Gr).max() < 1e-6
 
 
+def test_chunked_classes_vs_team_and_oracle(gpu, monkeypatch):
+    """The d-shell classes whose contracted block exceeds a lane's registers
+    ((12|11), (12|12), (22|11), (22|02)) run lane per quartet with the block
+    evaluated in chunks inside the group kernel; without the group kernel
+    (LIBRINT_NO_GROUP) they fall back to the team kernel.  Both builds, and
+    the chunked one with mixed precision, equal the oracle's screened build."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("chunked d classes vs team fallback, (H2O)4 even-tempered")
+    m = mol.water_cluster(4)
+    P = _rand_sym(m.nao(), 23)
+    Gr, cnt = refcpu.fock2e_screened(m, P, 1e-12)
+    scale = np.abs(Gr).max()
+    chunked = {(1, 2, 1, 1), (1, 2, 1, 2), (2, 2, 1, 1), (2, 2, 0, 2)}
+    out = {}
+    for mode in ("group", "flat"):
+        if mode == "flat":
+            monkeypatch.setenv("LIBRINT_NO_GROUP", "1")
+        ctx = FockContext(m, schwarz_tol=1e-12)
+        out[mode] = ctx.fock(P)
+        assert ctx.stats()["quartets"] == cnt
+        kern = {tuple(int(c[k]) for k in ("la", "lb", "lc", "ld")): c["kernel"] for c in ctx.class_stats()}
+        assert chunked <= set(kern)
+        for cl in chunked:
+            assert kern[cl].startswith("jk_group_kernel" if mode == "group" else "jk_team_kernel"), kern[cl]
+        if mode == "group":
+            ctx.set_precision(1, 1e-6)
+            out["mixed"] = ctx.fock(P)
+        ctx.close()
+    assert np.abs(out["group"] - Gr).max() < 1e-13 * scale
+    assert np.abs(out["flat"] - Gr).max() < 1e-13 * scale
+    assert np.abs(out["mixed"] - Gr).max() < 1e-6
+
+
 def test_mixed_precision_d_shells(gpu):
     from librint_amd import mol
     from librint_amd.context import FockContext
