@@ -30,15 +30,22 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix) dense peak
 HBM_PEAK = 8000.0         # GB/s
 
-# Counter-measured figures of the dominant kernel per launch, from the
-# committed rocprofv3 summaries (profiles/valu_counters_fock.md,
-# profiles/jk_pmc_traffic_fock.md), keyed by (workload, precision, kernel):
-#   TRAFFIC: HBM bytes, FETCH_SIZE x 2 + WRITE_SIZE (KiB, each scaled by what
-#            the counter reads for a copy of known size);
+# Counter-measured figures of the dominant kernel per launch (one build: the
+# sum over the launch's dispatches), from the committed rocprofv3 summary
+# profiles/valu_counters_fock.md, keyed by (workload, precision, kernel):
+#   TRAFFIC: HBM bytes, (FETCH_SIZE x 2 + WRITE_SIZE) x 1024 (KiB counters,
+#            each scaled by what it reads for a 256 MiB device copy: FETCH
+#            half the bytes read, WRITE all the bytes written);
 #   COUNTER_FLOPS: executed FP64 VALU flops, 64 x (2 FMA + ADD + MUL + TRANS)
 #            wavefront instructions.
-TRAFFIC = {}
-COUNTER_FLOPS = {}
+TRAFFIC = {
+    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): (2 * 1391941.9 + 2772292.6) * 1024,
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, false, false>"): (2 * 23506.1 + 149354.4) * 1024,
+}
+COUNTER_FLOPS = {
+    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): 6.7028e10,
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, false, false>"): 1.4862e10,
+}
 
 
 def core_guess_density(m):

@@ -280,25 +280,6 @@ __device__ __forceinline__ void rys_nodes_lds(T x, const T *rt, T *v, T *w)
     }
 }
 
-// Node and weight of root r only (run-time r), table block in LDS.
-template <typename T, int N>
-__device__ __forceinline__ void rys_one_lds(T x, const T *rt, int r, T &v, T &w)
-{
-    using R = RysTab<T, N, false>;
-    if (x >= (T)RYS_NSEG[N]) {
-        const double rs = rsqrt_pos((double)x);
-        v = (T)(RYS_HERM_NODE2[RYS_HOFF[N] + r] * (rs * rs));
-        w = (T)(RYS_HERM_W[RYS_HOFF[N] + r] * rs);
-        return;
-    }
-    const int k = (int)x;
-    const T s = (T)2 * (x - (T)k) - (T)1;
-    const T s2 = s * s, s4 = s2 * s2, s8 = s4 * s4;
-    const T *c = rt + k * R::STRIDE;
-    v = estrin_t<T>(c + r, R::NV, s, s2, s4, s8);
-    w = estrin_t<T>(c + N + r, R::NV, s, s2, s4, s8);
-}
-
 template <int LA, int LB, int LC, int LD, typename T = double>
 struct RysLds {
     using C = Class<LA, LB, LC, LD>;
@@ -357,23 +338,11 @@ __device__ __forceinline__ void prim_quartet_lds(T *eri, const PrimPair &p, cons
     const T pq[3] = {(T)p.pbx - qd[0], (T)p.pby - qd[1], (T)p.pbz - qd[2]};
     const T rr = pq[0] * pq[0] + pq[1] * pq[1] + pq[2] * pq[2];
     const T fac1 = (T)p.K * (T)q.K * rsq;
-    if constexpr (SD >= 0) {
-        // chunked classes: one root at a time (rolled), so that only one
-        // root's recurrence values are live
-        const T x = a0 * rr;
-#pragma unroll 1
-        for (int r = 0; r < N; r++) {
-            T vr, wr;
-            rys_one_lds<T, N>(x, rt, r, vr, wr);
-            root_term<LA, LB, LC, LD, T, SD, S0>(eri, p, q, pq, aij, akl, rs, fac1, vr, wr, rab, rcd);
-        }
-    } else {
-        T v[N], w[N];
-        rys_nodes_lds<T, N, RysLds<LA, LB, LC, LD, T>::WONLY>(a0 * rr, rt, v, w);
+    T v[N], w[N];
+    rys_nodes_lds<T, N, RysLds<LA, LB, LC, LD, T>::WONLY>(a0 * rr, rt, v, w);
 #pragma unroll
-        for (int r = 0; r < N; r++)
-            root_term<LA, LB, LC, LD, T, SD, S0>(eri, p, q, pq, aij, akl, rs, fac1, v[r], w[r], rab, rcd);
-    }
+    for (int r = 0; r < N; r++)
+        root_term<LA, LB, LC, LD, T, SD, S0>(eri, p, q, pq, aij, akl, rs, fac1, v[r], w[r], rab, rcd);
 }
 
 // One primitive quartet, all roots, accumulated into eri (double precision).
