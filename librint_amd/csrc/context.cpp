@@ -155,6 +155,7 @@ Context::~Context()
     dev_free(d_shell_slot_);
     dev_free(d_slot_coef_);
     dev_free(d_gtasks1_);
+    dev_free(d_gtasks0_);
     dev_free(d_hlscr_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -436,6 +437,7 @@ void Context::build_tasks()
     // the lists below replace device buffers that launches of an earlier
     // build (on this context's extra streams too) may still be reading
     if (d_tasks_) check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    gtasks_valid_ = false;  // the adjoint's lists derive from these tasks
     const int nb = (int)bins_.size();
     launches_.clear();
     stats_ = Stats();
@@ -1253,6 +1255,24 @@ void Context::build_grad_tasks()
         glaunch1_.push_back(G);
     }
     if (!all.empty()) d_gtasks1_ = dev_upload(all);
+    // bra side: the Fock tasks, with those of the chunked classes (64-ket
+    // windows in the Fock build) cut into the short windows the adjoint's
+    // one-ket-at-a-time tasks balance with
+    glaunch0_.clear();
+    dev_free(d_gtasks0_);
+    std::vector<JKTask> all0;
+    for (const ClassLaunch &L : launches_) {
+        GradLaunch G{L.la, L.lb, L.lc, L.ld, (int)all0.size(), 0};
+        if (class_split(L.la, L.lb, L.lc, L.ld) && lane_path(L.la, L.lb, L.lc, L.ld, L.fp32))
+            for (int t = L.first; t < L.first + L.count; t++) {
+                const JKTask &T = h_tasks_[t];
+                for (int k0 = 0; k0 < T.nket; k0 += team_kets_)
+                    all0.push_back(JKTask{T.bra, T.ket0 + k0, std::min(team_kets_, T.nket - k0), T.prims});
+            }
+        G.count = (int)all0.size() - G.first;
+        glaunch0_.push_back(G);
+    }
+    if (!all0.empty()) d_gtasks0_ = dev_upload(all0);
     gtasks_valid_ = true;
 }
 
@@ -1324,7 +1344,14 @@ void Context::grad2e(const double *P, double *gatom, double *denv, size_t env_l)
                   "grad launch");
         }
     };
-    for (const ClassLaunch &L : launches_) grad_launch(L.la, L.lb, L.lc, L.ld, 0, d_tasks_ + L.first, L.count);
+    for (size_t k = 0; k < launches_.size(); k++) {
+        const ClassLaunch &L = launches_[k];
+        const GradLaunch &G = glaunch0_[k];
+        if (G.count)
+            grad_launch(L.la, L.lb, L.lc, L.ld, 0, d_gtasks0_ + G.first, G.count);
+        else
+            grad_launch(L.la, L.lb, L.lc, L.ld, 0, d_tasks_ + L.first, L.count);
+    }
     for (const GradLaunch &G : glaunch1_) grad_launch(G.la, G.lb, G.lc, G.ld, 1, d_gtasks1_ + G.first, G.count);
     for (int i = 0; i < nst - 1; i++) {
         check(hipEventRecord(evj_[i], side_[i]), "hipEventRecord");

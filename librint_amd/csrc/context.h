@@ -205,6 +205,11 @@ private:
     };
     std::vector<GradLaunch> glaunch1_;
     JKTask *d_gtasks1_ = nullptr;
+    // bra side of the chunked classes: their Fock tasks (64-ket windows) cut
+    // into team_kets_ windows, since the adjoint kernel takes a task's kets
+    // one at a time (count 0: the class uses its Fock tasks as they are)
+    std::vector<GradLaunch> glaunch0_;
+    JKTask *d_gtasks0_ = nullptr;
     bool gtasks_valid_ = false;
     void build_grad_tasks();
     // classes with an f or g shell (hl.hip): scratch of one contracted block
