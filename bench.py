@@ -238,6 +238,37 @@ def bench_grad(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_scf(args):
+    """Full RHF on the GPU (core-Hamiltonian guess, DIIS, density and energy
+    criteria), FP64 and with mixed precision, on one workload: iterations,
+    wall time, time per iteration and the energy difference."""
+    from librint_amd.context import FockContext
+    m, tau = workload(args.workload)
+    out = []
+    for label, tau_mix in [("fp64", 0.0)] + [("mixed", t) for t in args.scf_tau_mix]:
+        ctx = FockContext(m, schwarz_tol=tau)
+        if tau_mix > 0.0:
+            ctx.set_precision(1, tau_mix, 0.0)
+        ctx.scf(imax=2, conv=1e-6, guess=1, diis=True)  # warm-up: lists, streams, code objects
+        t0 = time.perf_counter()
+        E, _, it = ctx.scf(imax=100, conv=1e-6, guess=1, diis=True, conv_e=1e-8)
+        wall = time.perf_counter() - t0
+        ctx.close()
+        out.append({"precision": label, "tau_mix": tau_mix, "energy_Eh": E, "iterations": it, "wall_s": wall,
+                    "ms_per_iteration": wall / max(it, 1) * 1e3})
+    for r in out[1:]:
+        r["energy_error_vs_fp64_gpu_Eh"] = r["energy_Eh"] - out[0]["energy_Eh"]
+    line = {
+        "metric": "RHF SCF on the GPU (core-H guess, DIIS, ||dP||/2 <= 1e-6 and |dE| <= 1e-8) s",
+        "value": out[0]["wall_s"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": out[0]["wall_s"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic seeded geometry",
+        "config": {"workload": args.workload, "nao": m.nao(), "schwarz": tau},
+        "runs": out,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def timed_builds(ctx, Pd, Gd, steps, warmup):
     """Wall ms per Fock build over `steps` builds after `warmup`, bracketed by
     device synchronisation."""
@@ -355,6 +386,10 @@ def main():
                     help="time the analytic RHF gradient (adjoint ERI kernel) instead of the Fock build")
     ap.add_argument("--copy-mb", type=float, default=0.0,
                     help="only copy this many MiB device to device --steps times (PMC calibration)")
+    ap.add_argument("--scf", action="store_true",
+                    help="time a full RHF (core-H guess, DIIS) in FP64 and mixed precision instead")
+    ap.add_argument("--scf-tau-mix", type=float, nargs="*", default=[1e-5],
+                    help="with --scf: tau_mix values of the mixed-precision runs")
     ap.add_argument("--fd", default="0",
                     help="with --grad: check 'all' or N seeded parameters against re-converged central FD")
     args = ap.parse_args()
@@ -368,6 +403,8 @@ def main():
 
     if args.grad:
         return bench_grad(args)
+    if args.scf:
+        return bench_scf(args)
     if args.copy_mb > 0:
         # device-to-device copy of a known size: the reference point for the
         # FETCH_SIZE / WRITE_SIZE counters of the roofline traffic figure
