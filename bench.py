@@ -39,11 +39,11 @@ HBM_PEAK = 8000.0         # GB/s
 #   COUNTER_FLOPS: executed FP64 VALU flops, 64 x (2 FMA + ADD + MUL + TRANS)
 #            wavefront instructions.
 TRAFFIC = {
-    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): (2 * 1391941.9 + 2772292.6) * 1024,
-    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, false, false>"): (2 * 23506.1 + 149354.4) * 1024,
+    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): (2 * 1421109.1 + 2613581.6) * 1024,
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, false, false>"): (2 * 23539.0 + 149644.3) * 1024,
 }
 COUNTER_FLOPS = {
-    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): 6.7028e10,
+    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): 6.6876e10,
     ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, false, false>"): 1.4862e10,
 }
 

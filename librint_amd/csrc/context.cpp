@@ -432,6 +432,26 @@ bool Context::lane_path(int la, int lb, int lc, int ld, int mixed) const
     return !no_group_ && jk_group_lds(la, lb, lc, ld, mixed ? 1 : 0, nao_, false) <= (size_t)max_lds_;
 }
 
+// Staging the bra's density rows in LDS (DROW) doubles the row space per
+// workgroup but turns the exchange update's gathers of D into LDS reads.
+// Default (mode 2): stage them where a quartet has fewer than 4 primitive
+// quartets on average (the exchange update is then a large part of the
+// work) and the rows do not cost occupancy: two 4-wave workgroups fit a CU,
+// or the class runs 8- or 12-wave workgroups (one of them holds the 2 or 3
+// waves per SIMD its registers allow).  LIBRINT_DROW=0 never, 1 wherever it
+// fits.  Measured: (H2O)32 48.5 -> 46.8 ms, the contracted C16H34 lists
+// unchanged (none qualifies; staging them all was slower, 1.62 -> 1.65 ms).
+bool Context::drow_plan(int la, int lb, int lc, int ld, int fp32, double prim_quartets, double quartets) const
+{
+    const size_t lds = jk_group_lds(la, lb, lc, ld, fp32, nao_, false);
+    if (no_group_ || lds > (size_t)max_lds_) return false;  // no group kernel
+    const size_t lds_drow = jk_group_lds(la, lb, lc, ld, fp32, nao_, true);
+    if (lds_drow > (size_t)max_lds_) return false;
+    if (drow_mode_ != 2) return drow_mode_ == 1;
+    return prim_quartets < 4.0 * quartets &&
+           (lds_drow <= (size_t)max_lds_ / 2 || jk_group_waves(la, lb, lc, ld, true) > 4);
+}
+
 void Context::build_tasks()
 {
     // the lists below replace device buffers that launches of an earlier
@@ -528,19 +548,29 @@ void Context::build_tasks()
         // about (NFA+NFB)*nao/64 tasks for the flush to stay small next to its
         // quartets; at most 256, and small enough to leave >= 4096 groups per
         // launch for load balance (never below 8).
-        int cap = GROUP_MAX_TASKS;
+        int gla, glb, glc, gld;
+        cls_ls(std::get<0>(kv.first), gla, glb);
+        cls_ls(std::get<1>(kv.first), glc, gld);
+        const int gfp32 = mixed && class_has_fp32(gla, glb, glc, gld) && lane_path(gla, glb, glc, gld, mixed) ? 1 : 0;
+        const bool gdrow = class_has_fp32(gla, glb, glc, gld) &&
+                           drow_plan(gla, glb, glc, gld, gfp32, (double)A.stat.prim_quartets, (double)A.stat.quartets);
+        // a wavefront of the group kernel holds the classification flags of
+        // its share of a group's tasks, one per lane: at most 64 x (its
+        // workgroup's wavefronts) tasks per group
+        const int gmax = 64 * (gdrow ? jk_group_waves(gla, glb, glc, gld, true) : 4);
+        int cap = GROUP_MAX_TASKS > 0 ? std::min(GROUP_MAX_TASKS, gmax) : 0;
         if (cap <= 0) {
-            int la, lb;
-            cls_ls(std::get<0>(kv.first), la, lb);
-            const int rows = ncart(la) + ncart(lb);
+            const int rows = ncart(gla) + ncart(glb);
             // lists of cheap quartets (fewer than 4 primitive quartets each,
-            // the uncontracted water-cluster lists) take groups 4 times as
-            // large: there the per-group row staging, zeroing and flush
-            // weigh on quartets that cost little (measured on (H2O)32,
-            // one stream: 46.4 -> 43.0 ms with 256-task groups)
+            // the uncontracted water-cluster lists) take groups up to 12
+            // times as large: there the per-group row staging, zeroing and
+            // flush weigh on quartets that cost little (measured on (H2O)32,
+            // one stream: 46.4 -> 43.0 ms with 256-task groups; with the
+            // density rows in LDS and up to 768-task groups 36.35 -> 36.03 ms)
             const bool cheap = (double)A.stat.prim_quartets < 4.0 * (double)A.stat.quartets;
-            cap = std::min(256, std::max(8, (rows * nao_ + 63) / 64 * (cheap ? 4 : 1)));
+            cap = std::min(gmax, std::max(8, (rows * nao_ + 63) / 64 * (cheap ? (gdrow ? 12 : 4) : 1)));
             cap = std::max(8, std::min(cap, (int)(A.tasks.size() / 4096)));
+            cap = std::min(cap, gmax);
         }
         for (size_t t = 0; t < A.tasks.size();) {
             const int bra = A.tasks[t].bra;
@@ -613,7 +643,8 @@ void Context::build_tasks()
         L.lb = lb;
         L.lc = lc;
         L.ld = ld;
-        L.fp32 = mixed && class_has_fp32(la, lb, lc, ld) && lane_path(la, lb, lc, ld, mixed) ? 1 : 0;
+        L.fp32 = gfp32;
+        L.drow_plan = gdrow ? 1 : 0;
         L.first = (int)tasks.size();
         L.gfirst = (int)groups.size();
         for (auto &g : A.groups) {
@@ -925,23 +956,9 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
         check(hipEventRecord(evs_[k], sk), "hipEventRecord");
         const size_t lds = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, false);
         const bool grouped = !no_group_ && lds <= (size_t)max_lds_;
-        // Staging the bra's density rows in LDS (DROW) doubles the row space
-        // per workgroup but turns the exchange update's gathers of D into LDS
-        // reads.  Default (mode 2): stage them where a quartet has fewer than
-        // 4 primitive quartets on average (the exchange update is then a large
-        // part of the work) and two workgroups still fit a CU; LIBRINT_DROW=0
-        // never, 1 wherever it fits.  Measured: (H2O)32 48.5 -> 46.8 ms, the
-        // contracted C16H34 lists unchanged (none qualifies; staging them all
-        // was slower, 1.62 -> 1.65 ms).
-        const size_t lds_drow = jk_group_lds(L.la, L.lb, L.lc, L.ld, L.fp32, nao_, true);
-        bool drow = drow_mode_ == 1;
-        // the rows in LDS must not cost occupancy: two 4-wave workgroups fit
-        // a CU, or the class runs 8- or 12-wave workgroups (one of them
-        // holds the 2 or 3 waves per SIMD its registers allow)
-        if (drow_mode_ == 2)
-            drow = (double)L.prim_quartets < 4.0 * (double)L.quartets &&
-                   (lds_drow <= (size_t)max_lds_ / 2 || jk_group_waves(L.la, L.lb, L.lc, L.ld, true) > 4);
-        drow = drow && grouped && lds_drow <= (size_t)max_lds_;
+        // the density-row staging planned with the task lists (drow_plan):
+        // their group sizes assume its workgroup size
+        const bool drow = grouped && L.drow_plan;
         L.drow = drow ? 1 : 0;
         const bool hl = hl_class(L.la, L.lb, L.lc, L.ld);
         L.variant = hl ? 3

@@ -46,6 +46,7 @@ struct ClassLaunch {
     long long jbase = -1;  // Coulomb slab offset in the whole-build layout (-1: global atomics)
     int cfirst = 0, ccount = 0;  // range in the slab chunks (one chunk when the slabs are kept whole)
     int drow = 0;                // bra density rows staged in LDS in the last build
+    int drow_plan = 0;           // ... as planned with the task lists (group sizes follow it)
 };
 
 class Context {
@@ -118,6 +119,7 @@ private:
     void compute_schwarz();
     void build_tasks();
     bool lane_path(int la, int lb, int lc, int ld, int mixed) const;
+    bool drow_plan(int la, int lb, int lc, int ld, int fp32, double prim_quartets, double quartets) const;
     void upload();
     void ensure_side_streams(int n);
     struct FockScratch {
