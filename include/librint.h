@@ -165,13 +165,15 @@ int librint_schwarz(librint_ctx *ctx, double *out);
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 
-/* Per angular-class launch of the current plan, 13 values each:
+/* Per angular-class launch of the current plan, 14 values each:
  * la, lb, lc, ld, quartets, primitive quartets, algorithmic flops, lanes
  * launched (64 x tasks), device ms of the launch in the last host-side build
  * (librint_fock), task count, mixed (1: the launch picks FP32 recurrences per
  * task in mixed precision), kernel of the last build (0 jk_kernel,
  * 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel for classes with an f or
- * g shell, -1 none yet), bra density rows staged in LDS (0/1).
+ * g shell, -1 none yet), bra density rows staged in LDS (0/1), device ms of
+ * the Coulomb-slab reductions inside the launch's time (launches whose slabs
+ * run in chunks reduce each chunk right after it).
  * Returns the number of launches (out may be NULL to query it); writes at
  * most n launches. */
 int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);

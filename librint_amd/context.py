@@ -101,11 +101,11 @@ class FockContext:
     def class_stats(self):
         """Per angular-class launch: counts, flops and the last build's device ms."""
         n = library.librint_ctx_class_stats(self.h, None, 0)
-        out = np.zeros(13 * max(n, 1))
+        out = np.zeros(14 * max(n, 1))
         library.librint_ctx_class_stats(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
         keys = ("la", "lb", "lc", "ld", "quartets", "prim_quartets", "flops", "lanes", "ms", "tasks", "mixed",
-                "variant", "drow")
-        rows = [dict(zip(keys, out[13 * k:13 * k + 13].tolist())) for k in range(n)]
+                "variant", "drow", "reduce_ms")
+        rows = [dict(zip(keys, out[14 * k:14 * k + 14].tolist())) for k in range(n)]
         for r in rows:
             v = int(r["variant"])
             ls = "%d, %d, %d, %d" % (r["la"], r["lb"], r["lc"], r["ld"])

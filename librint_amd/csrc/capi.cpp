@@ -695,11 +695,11 @@ int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n)
     int k = 0;
     for (; k < (int)L.size() && k < n; k++) {
         const ClassLaunch &c = L[k];
-        const double v[13] = {(double)c.la,        (double)c.lb,    (double)c.lc,      (double)c.ld,
+        const double v[14] = {(double)c.la,        (double)c.lb,    (double)c.lc,      (double)c.ld,
                               (double)c.quartets,  (double)c.prim_quartets, c.flops,  (double)c.lanes,
                               (double)c.ms,        (double)c.count, (double)c.fp32,    (double)c.variant,
-                              (double)c.drow};
-        for (int j = 0; j < 13; j++) out[13 * k + j] = v[j];
+                              (double)c.drow,      (double)c.reduce_ms};
+        for (int j = 0; j < 14; j++) out[14 * k + j] = v[j];
     }
     return (int)L.size();
 }

@@ -161,6 +161,8 @@ Context::~Context()
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evc_) (void)hipEventDestroy(e);
     for (hipEvent_t e : evs_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evr0_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evr1_) (void)hipEventDestroy(e);
     for (hipEvent_t e : evj_) (void)hipEventDestroy(e);
     if (evf_) (void)hipEventDestroy(evf_);
     for (hipStream_t q : side_) (void)hipStreamDestroy(q);
@@ -760,6 +762,13 @@ void Context::build_tasks()
     evs_.assign(launches_.size(), nullptr);
     for (auto &e : evc_) check(hipEventCreate(&e), "hipEventCreate");
     for (auto &e : evs_) check(hipEventCreate(&e), "hipEventCreate");
+    for (hipEvent_t e : evr0_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evr1_) (void)hipEventDestroy(e);
+    evr0_.assign(chunks_.size(), nullptr);
+    evr1_.assign(chunks_.size(), nullptr);
+    red_rec_.assign(chunks_.size(), 0);
+    for (auto &e : evr0_) check(hipEventCreate(&e), "hipEventCreate");
+    for (auto &e : evr1_) check(hipEventCreate(&e), "hipEventCreate");
     dev_free(d_tasks_);
     d_tasks_ = dev_upload(tasks);
     dev_free(d_groups_);
@@ -917,6 +926,7 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
     }
     classified_ = classify;
     mixed_last_ = mixed;
+    if (sc) std::fill(red_rec_.begin(), red_rec_.end(), 0);  // no slabs, no reductions
     const unsigned char *flags = classify ? d_flags_ : nullptr;
     const unsigned char *gmask = classify ? d_gmask_ : nullptr;
     check(hipEventRecord(ev0_, st), "hipEventRecord");
@@ -988,10 +998,15 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
                     check(launch_jk(L.la, L.lb, L.lc, L.ld, L.fp32, d_tasks_ + C.tfirst, C.tcount, ctr,
                                     flags ? flags + C.tfirst : nullptr, 3, B, D_dev, M, nao_, js, 0, sk),
                           "jk launch");
-                if (js && !jslab_whole_ && C.icount)
+                if (!sc) red_rec_[ci] = 0;
+                if (js && !jslab_whole_ && C.icount) {
+                    check(hipEventRecord(evr0_[ci], sk), "hipEventRecord");
                     check(launch_jreduce(d_items_ + C.ifirst, C.icount, ncart(L.lc) * ncart(L.ld), d_item_tasks_,
                                          d_tasks_, flags, 3, jbuf, B, M, nao_, sk),
                           "coulomb reduction launch");
+                    check(hipEventRecord(evr1_[ci], sk), "hipEventRecord");
+                    red_rec_[ci] = 1;
+                }
             }
         }
         check(hipEventRecord(evc_[k], sk), "hipEventRecord");
@@ -1037,8 +1052,14 @@ void Context::fock_host(const double *D, double *G)
 void Context::collect_class_times()
 {
     for (size_t k = 0; k < launches_.size(); k++) {
+        ClassLaunch &L = launches_[k];
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, evs_[k], evc_[k]) == hipSuccess) launches_[k].ms = ms;
+        if (hipEventElapsedTime(&ms, evs_[k], evc_[k]) == hipSuccess) L.ms = ms;
+        L.reduce_ms = 0.f;
+        for (int ci = L.cfirst; ci < L.cfirst + L.ccount && ci < (int)red_rec_.size(); ci++) {
+            float r = 0.f;
+            if (red_rec_[ci] && hipEventElapsedTime(&r, evr0_[ci], evr1_[ci]) == hipSuccess) L.reduce_ms += r;
+        }
     }
 }
 

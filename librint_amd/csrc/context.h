@@ -42,6 +42,7 @@ struct ClassLaunch {
     long long quartets = 0, prim_quartets = 0, lanes = 0;
     double flops = 0;
     float ms = 0.f;    // device time of this launch in the last build
+    float reduce_ms = 0.f;  // ... of which its slab chunks' Coulomb reductions
     int variant = -1;  // kernel of the last build: 0 jk_kernel, 1 jk_group_kernel, 2 jk_team_kernel, 3 hl_kernel
     long long jbase = -1;  // Coulomb slab offset in the whole-build layout (-1: global atomics)
     int cfirst = 0, ccount = 0;  // range in the slab chunks (one chunk when the slabs are kept whole)
@@ -225,6 +226,9 @@ private:
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     std::vector<hipEvent_t> evc_;  // one event after each class launch
     std::vector<hipEvent_t> evs_;  // one event before each class launch (same stream)
+    // around each slab chunk's Coulomb reduction (inside its launch's events)
+    std::vector<hipEvent_t> evr0_, evr1_;
+    std::vector<char> red_rec_;    // chunk's reduction recorded in the last build
     // class launches after the dominant one are dealt round-robin over the
     // caller's stream and nstreams_ - 1 extra streams (LIBRINT_STREAMS),
     // joined before the Coulomb reduction: small launches and launch tails
