@@ -158,10 +158,11 @@ size_t librint_int2e_shells(librint_ctx *ctx, const int *quartets, int nq, doubl
  * (nbas*(nbas+1)/2 doubles). Returns 0 on success. */
 int librint_schwarz(librint_ctx *ctx, double *out);
 
-/* Counters of the current quartet plan, out[0..8]:
+/* Counters of the current quartet plan, out[0..9]:
  * quartets, primitive quartets, contracted integrals, algorithmic flops,
  * candidate quartets, kernel launches, device ms of the last build, Schwarz tol,
- * device ms of the last adjoint (gradient) kernel launches.
+ * device ms of the last adjoint (gradient) kernel launches, densities the last
+ * SCF on the context took from the purified projector instead of eigh.
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 
@@ -178,10 +179,6 @@ int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
  * most n launches. */
 int librint_ctx_class_stats(const librint_ctx *ctx, double *out, int n);
 
-/* RHF on the context.  guess: 0 identity (reference), 1 core Hamiltonian,
- * 2 caller density in P_inout.  diis: 0/1.  On return P_inout (nao*nao,
- * may be NULL) holds the final density and *iters the cycle count.
- * Returns the energy, NaN on failure. */
 /* RHF energy gradient at a converged density P (host, nao*nao):
  *   gnuc[3*natm] = dE/dR  (core H incl. the point-charge positions, the
  *                  two-electron adjoint, -1/2 tr(PFP dS), nuclear repulsion)
@@ -198,6 +195,15 @@ int librint_ctx_gradient(librint_ctx *ctx, const double *P, double *gnuc, double
  * screening.  No counterpart in the reference. */
 double *grad_nuc_c(int *atm, size_t atm_l, int *bas, size_t bas_l, double *env, size_t env_l, double *P, size_t P_l);
 
+/* RHF on the context.  guess: 0 identity (reference), 1 core Hamiltonian,
+ * 2 caller density in P_inout.  diis: 0/1.  On return P_inout (nao*nao,
+ * may be NULL) holds the final density and *iters the cycle count.
+ * Returns the energy, NaN on failure.  The reference mode (guess 0, no
+ * DIIS) diagonalises F in every cycle as the reference does; with DIIS and
+ * guess 1 or 2 each density is the occupied-space projector of F by
+ * trace-correcting purification (matrix products only), falling back to
+ * the eigensolver where that does not converge to a projector of trace
+ * nelec/2 (LIBRINT_SCF_EIGH=1: always the eigensolver). */
 double librint_scf(librint_ctx *ctx, int nelec, int imax, double conv, int guess, int diis, double *P_inout,
                    int *iters);
 

@@ -32,6 +32,7 @@ struct Stats {
     int launches = 0;
     float last_kernel_ms = 0.f;    // device time of the last build (events)
     float last_grad_ms = 0.f;      // device time of the last adjoint (grad2e) launches
+    int scf_purified = 0;          // densities by purification in the last SCF on the context
 };
 
 struct ClassLaunch {
@@ -107,6 +108,7 @@ public:
     void grad2e(const double *P, double *gatom, double *denv, size_t env_l);
 
     const Stats &stats() const { return stats_; }
+    void note_scf_purified(int k) { stats_.scf_purified = k; }
     const std::vector<Shell> &shells() const { return shells_; }
     const std::vector<int> &atm() const { return atm_; }
     const std::vector<double> &env() const { return env_; }

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <stdexcept>
 #include <string>
@@ -134,6 +135,82 @@ __global__ void antisym_kernel(int n, const double *__restrict__ A, double *__re
     e[i] = A[i] - A[(size_t)r * n + c];
 }
 
+// Gershgorin bounds of a symmetric n x n matrix (one workgroup): out[0] =
+// min_j (A_jj - r_j), out[1] = max_j (A_jj + r_j), r_j the off-diagonal
+// absolute column sum.
+__global__ __launch_bounds__(1024) void gersh_kernel(int n, const double *__restrict__ A, double *__restrict__ out)
+{
+    __shared__ double lo[1024], hi[1024];
+    double l = 1e300, h = -1e300;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        const double *c = A + (size_t)j * n;
+        double r = 0.0;
+        for (int i = 0; i < n; i++) r += fabs(c[i]);
+        const double d = c[j];
+        r -= fabs(d);
+        l = fmin(l, d - r);
+        h = fmax(h, d + r);
+    }
+    lo[threadIdx.x] = l;
+    hi[threadIdx.x] = h;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            lo[threadIdx.x] = fmin(lo[threadIdx.x], lo[threadIdx.x + s]);
+            hi[threadIdx.x] = fmax(hi[threadIdx.x], hi[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = lo[0];
+        out[1] = hi[0];
+    }
+}
+
+// X = (emax I - A) / (emax - emin): the spectrum mapped into [0, 1], the
+// lowest eigenvalues of A nearest 1
+__global__ void sp2_init_kernel(int n, const double *__restrict__ A, double emin, double emax, double *__restrict__ X)
+{
+    const size_t nn = (size_t)n * n;
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nn) return;
+    const size_t i = k % n, j = k / n;
+    X[k] = ((i == j ? emax : 0.0) - A[k]) / (emax - emin);
+}
+
+// out[0] = tr X, out[1] = tr X2 (one workgroup)
+__global__ __launch_bounds__(1024) void trace2_kernel(int n, const double *__restrict__ X, const double *__restrict__ X2,
+                                                      double *__restrict__ out)
+{
+    __shared__ double a[1024], b[1024];
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        s1 += X[(size_t)i * n + i];
+        s2 += X2[(size_t)i * n + i];
+    }
+    a[threadIdx.x] = s1;
+    b[threadIdx.x] = s2;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            a[threadIdx.x] += a[threadIdx.x + s];
+            b[threadIdx.x] += b[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = a[0];
+        out[1] = b[0];
+    }
+}
+
+// X = 2 X - X2
+__global__ void sp2_up_kernel(size_t m, double *__restrict__ X, const double *__restrict__ X2)
+{
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) X[k] = 2.0 * X[k] - X2[k];
+}
+
 __global__ void identity_kernel(int n, double *__restrict__ A)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -228,10 +305,63 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
     hipLaunchKernelGGL(scale_cols_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, T1.p, w.p, X.p);
     hck(hipGetLastError(), "scale_cols launch");
 
+    // The occupied-space projector of F' = X^T F X by trace-correcting
+    // purification (A. M. N. Niklasson, Phys. Rev. B 66, 155115 (2002)):
+    // the spectrum mapped into [0, 1] by its Gershgorin bounds, then
+    // X <- X^2 or 2X - X^2, whichever brings tr X nearer to nocc, until the
+    // idempotency error tr X - tr X^2 stops falling.  Matrix products only
+    // (rocBLAS dgemm), against the divide-and-conquer eigensolver's
+    // tridiagonal reduction.  Returns false (the caller diagonalises) when it
+    // does not settle on a projector of trace nocc: a closed or vanishing
+    // gap, where the aufbau occupation is not a projector's limit.
+    Dev SPa(nn), SPb(nn), sc2(2);
+    auto purify = [&](const double *Fo, double *Proj) -> bool {
+        double hb[2];
+        hipLaunchKernelGGL(gersh_kernel, dim3(1), dim3(1024), 0, 0, n, Fo, sc2.p);
+        hck(hipGetLastError(), "gersh launch");
+        hck(hipMemcpy(hb, sc2.p, sizeof(hb), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        if (!(hb[1] > hb[0])) return false;
+        hipLaunchKernelGGL(sp2_init_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, Fo, hb[0], hb[1], Proj);
+        hck(hipGetLastError(), "sp2 init launch");
+        double idem_hist[2] = {1e300, 1e300};
+        for (int k = 0; k < 120; k++) {
+            B.gemm(n, false, false, 1.0, Proj, Proj, 0.0, SPb.p);  // X^2
+            hipLaunchKernelGGL(trace2_kernel, dim3(1), dim3(1024), 0, 0, n, Proj, SPb.p, sc2.p);
+            hck(hipGetLastError(), "trace launch");
+            double t[2];
+            hck(hipMemcpy(t, sc2.p, sizeof(t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            const double idem = t[0] - t[1];
+            // converged: the error has reached rounding level, where it no
+            // longer falls over two steps (the stopping rule of Kruchinina,
+            // Rudberg and Rubensson, J. Chem. Theory Comput. 12, 5788 (2016))
+            if ((k >= 2 && idem < 1e-6 && idem >= idem_hist[1]) || idem <= 0.0)
+                return std::fabs(t[0] - nocc) < 1e-6;
+            idem_hist[1] = idem_hist[0];
+            idem_hist[0] = idem;
+            if (std::fabs(t[1] - nocc) < std::fabs(2.0 * t[0] - t[1] - nocc)) {
+                B.copy(nn, SPb.p, Proj);
+            } else {
+                hipLaunchKernelGGL(sp2_up_kernel, dim3(blocks(nn)), dim3(256), 0, 0, nn, Proj, SPb.p);
+                hck(hipGetLastError(), "sp2 update launch");
+            }
+        }
+        return false;
+    };
+    // LIBRINT_SCF_EIGH=1 diagonalises in every iteration of the DIIS mode too
+    const bool use_purify = o.diis && o.guess != 0 && !getenv("LIBRINT_SCF_EIGH");
+    int npurified = 0;
+
     // P from F: F' = X^T F X, eigenvectors C', C = X C', P = 2 C_occ C_occ^T
+    // (or, in the DIIS mode, P = 2 X Proj X^T with the purified projector)
     auto density_from_F = [&](const double *Fm) {
         B.gemm(n, false, false, 1.0, Fm, X.p, 0.0, T1.p);   // F X
         B.gemm(n, true, false, 1.0, X.p, T1.p, 0.0, T2.p);  // X^T F X
+        if (use_purify && purify(T2.p, SPa.p)) {
+            B.gemm(n, false, false, 1.0, X.p, SPa.p, 0.0, T1.p);  // X Proj
+            B.gemm(n, false, true, 2.0, T1.p, X.p, 0.0, P.p);     // 2 X Proj X^T
+            npurified++;
+            return;
+        }
         B.eigh(n, T2.p, w.p, e.p, info);
         B.gemm(n, false, false, 1.0, X.p, T2.p, 0.0, T1.p);  // C
         B.gemm(n, false, true, 2.0, T1.p, T1.p, 0.0, P.p, nocc);
@@ -321,6 +451,7 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
         if (o.trace) std::fprintf(stderr, "librint scf iter %d delta %.3e\n", it, delta);
     }
     out.iters = it;
+    ctx.note_scf_purified(npurified);
     out.delta = delta;
     out.dE = dEn;
     out.P.resize(nn);

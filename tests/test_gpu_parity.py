@@ -439,6 +439,32 @@ def test_mixed_precision_d_shells(gpu):
     assert abs(0.5 * np.sum(P * (Gm - G64))) < 1e-6
 
 
+@pytest.mark.parametrize("builder", ["w4", "c16h34"])
+def test_scf_purification_matches_eigh(gpu, builder, monkeypatch):
+    """In the DIIS mode the SCF takes each density from the purified
+    occupied-space projector instead of diagonalising F; it converges to the
+    same energy (1e-10 Eh) and density in the same number of cycles as the
+    eigensolver path (LIBRINT_SCF_EIGH=1)."""
+    from librint_amd import mol
+    from librint_amd.context import FockContext
+    _say("SCF purification vs eigh", builder)
+    m = {"w4": lambda: mol.water_cluster(4), "c16h34": mol.c16h34_631g}[builder]()
+    out = {}
+    for mode in ("purify", "eigh"):
+        if mode == "eigh":
+            monkeypatch.setenv("LIBRINT_SCF_EIGH", "1")
+        ctx = FockContext(m, schwarz_tol=1e-12)
+        E, P, it = ctx.scf(imax=100, conv=1e-6, guess=1, diis=True, conv_e=1e-8)
+        out[mode] = (E, P, it, ctx.stats()["scf_purified"])
+        ctx.close()
+    (E1, P1, it1, n1), (E2, P2, it2, n2) = out["purify"], out["eigh"]
+    _say("E", E1, E2, "iterations", it1, it2, "purified", n1, n2)
+    assert n2 == 0 and n1 >= it1  # every density of the purifying run (guess and cycles)
+    assert abs(E1 - E2) < 1e-10
+    assert abs(it1 - it2) <= 1
+    assert np.abs(P1 - P2).max() < 1e-6
+
+
 def test_mixed_precision_energy_vs_oracle(gpu):
     """Mixed-precision two-electron energies against the oracle's CPU build
     (Schwarz-screened restatement of integral2e_fock, FP64) on the (H2O)4
