@@ -324,6 +324,7 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
         hipLaunchKernelGGL(sp2_init_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, Fo, hb[0], hb[1], Proj);
         hck(hipGetLastError(), "sp2 init launch");
         double idem_hist[2] = {1e300, 1e300};
+        const double idem_floor = std::max(1e-11, 1e-14 * n);
         for (int k = 0; k < 120; k++) {
             B.gemm(n, false, false, 1.0, Proj, Proj, 0.0, SPb.p);  // X^2
             hipLaunchKernelGGL(trace2_kernel, dim3(1), dim3(1024), 0, 0, n, Proj, SPb.p, sc2.p);
@@ -333,8 +334,11 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
             const double idem = t[0] - t[1];
             // converged: the error has reached rounding level, where it no
             // longer falls over two steps (the stopping rule of Kruchinina,
-            // Rudberg and Rubensson, J. Chem. Theory Comput. 12, 5788 (2016))
-            if ((k >= 2 && idem < 1e-6 && idem >= idem_hist[1]) || idem <= 0.0)
+            // Rudberg and Rubensson, J. Chem. Theory Comput. 12, 5788 (2016)),
+            // and is below idem_floor, so that every eigenvalue of the
+            // projector is within about 1e-11 of 0 or 1 (a looser stop left
+            // the SCF density unable to settle below |dP| ~ 1e-9)
+            if ((k >= 2 && idem < idem_floor && idem >= idem_hist[1]) || idem <= 0.0)
                 return std::fabs(t[0] - nocc) < 1e-6;
             idem_hist[1] = idem_hist[0];
             idem_hist[0] = idem;
@@ -352,11 +356,15 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
     int npurified = 0;
 
     // P from F: F' = X^T F X, eigenvectors C', C = X C', P = 2 C_occ C_occ^T
-    // (or, in the DIIS mode, P = 2 X Proj X^T with the purified projector)
-    auto density_from_F = [&](const double *Fm) {
+    // (or, in the DIIS mode while the density still changes by more than
+    // 100 conv, P = 2 X Proj X^T with the purified projector: the projector
+    // is idempotent to rounding level but not to the eigenvectors' 1e-15,
+    // and X = U s^{-1/2} magnifies what is left, so the last cycles, which
+    // decide the converged density, diagonalise)
+    auto density_from_F = [&](const double *Fm, double dlast) {
         B.gemm(n, false, false, 1.0, Fm, X.p, 0.0, T1.p);   // F X
         B.gemm(n, true, false, 1.0, X.p, T1.p, 0.0, T2.p);  // X^T F X
-        if (use_purify && purify(T2.p, SPa.p)) {
+        if (use_purify && dlast > 100.0 * conv && purify(T2.p, SPa.p)) {
             B.gemm(n, false, false, 1.0, X.p, SPa.p, 0.0, T1.p);  // X Proj
             B.gemm(n, false, true, 2.0, T1.p, X.p, 0.0, P.p);     // 2 X Proj X^T
             npurified++;
@@ -370,7 +378,7 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
         hipLaunchKernelGGL(identity_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, P.p);
         hck(hipGetLastError(), "identity launch");
     } else if (o.guess == 1) {
-        density_from_F(H.p);
+        density_from_F(H.p, 1.0);
     } else {
         if (!o.P0) throw std::invalid_argument("guess=2 needs a starting density");
         hck(hipMemcpy(P.p, o.P0, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
@@ -444,7 +452,7 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
                 }
             }
         }
-        density_from_F(F.p);
+        density_from_F(F.p, delta);
         B.axpy(nn, -1.0, P.p, Pold.p);
         delta = B.nrm2(nn, Pold.p) / 2.0;
         it++;

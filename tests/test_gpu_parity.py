@@ -441,10 +441,11 @@ def test_mixed_precision_d_shells(gpu):
 
 @pytest.mark.parametrize("builder", ["w4", "c16h34"])
 def test_scf_purification_matches_eigh(gpu, builder, monkeypatch):
-    """In the DIIS mode the SCF takes each density from the purified
-    occupied-space projector instead of diagonalising F; it converges to the
-    same energy (1e-10 Eh) and density in the same number of cycles as the
-    eigensolver path (LIBRINT_SCF_EIGH=1)."""
+    """In the DIIS mode the SCF takes its densities from the purified
+    occupied-space projector instead of diagonalising F until the density
+    changes by less than 100 conv; it converges to the same energy (1e-10 Eh)
+    and density in the same number of cycles as the eigensolver path
+    (LIBRINT_SCF_EIGH=1)."""
     from librint_amd import mol
     from librint_amd.context import FockContext
     _say("SCF purification vs eigh", builder)
@@ -459,7 +460,7 @@ def test_scf_purification_matches_eigh(gpu, builder, monkeypatch):
         ctx.close()
     (E1, P1, it1, n1), (E2, P2, it2, n2) = out["purify"], out["eigh"]
     _say("E", E1, E2, "iterations", it1, it2, "purified", n1, n2)
-    assert n2 == 0 and n1 >= it1  # every density of the purifying run (guess and cycles)
+    assert n2 == 0 and 3 <= n1 <= it1  # the guess and the early cycles purified
     assert abs(E1 - E2) < 1e-10
     assert abs(it1 - it2) <= 1
     assert np.abs(P1 - P2).max() < 1e-6
