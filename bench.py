@@ -253,9 +253,10 @@ def bench_scf(args):
         t0 = time.perf_counter()
         E, _, it = ctx.scf(imax=100, conv=1e-6, guess=1, diis=True, conv_e=1e-8)
         wall = time.perf_counter() - t0
+        purified = int(ctx.stats()["scf_purified"])
         ctx.close()
         out.append({"precision": label, "tau_mix": tau_mix, "energy_Eh": E, "iterations": it, "wall_s": wall,
-                    "ms_per_iteration": wall / max(it, 1) * 1e3})
+                    "ms_per_iteration": wall / max(it, 1) * 1e3, "purified_densities": purified})
     for r in out[1:]:
         r["energy_error_vs_fp64_gpu_Eh"] = r["energy_Eh"] - out[0]["energy_Eh"]
     line = {
