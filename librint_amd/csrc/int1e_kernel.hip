@@ -10,6 +10,8 @@
 // Rys quadrature with the same tables as the ERI kernels.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "int1e_gpu.h"
 #include "rys_eval.h"
 
@@ -158,8 +160,40 @@ __global__ __launch_bounds__(128) void int1e_kernel(const int2 *__restrict__ pai
 // d/dC V^C = -(d/dA + d/dB) V^C.  Per primitive x_A^a exp(-alpha r_A^2):
 //   d/dA_x = 2 alpha G(a + 1_x) - a_x G(a - 1_x),  d/dalpha = -sum_x G(a + 2_x),
 //   d/dc = G / c.
-// One thread per ordered shell pair (per pair and point charge for V); the
-// 1D tables hold bra powers up to LI + 2.
+// The 1D tables hold bra powers up to LI + 2.
+//
+// S and T: one thread per ordered shell pair.  V: a segment of CW = 2^k
+// lanes per pair (CW = the point-charge count rounded up to a power of two,
+// at most 64); lane cs of a segment takes the charges cs, cs + CW, ...  A
+// segment shares its pair's primitive loops, so the exponent and coefficient
+// derivatives of a bra primitive and the pair's bra-centre gradient are
+// summed over the segment by lane exchanges before one atomic; with one
+// charge per lane (at most 64 charges) each charge's gradient is summed over
+// the pair's primitives in registers and over the wavefront's segments
+// before one atomic per charge and wavefront.
+
+// cartesian exponents of component f of a shell of angular momentum l, in
+// the order of C1E (x descending, then y descending)
+__host__ __device__ constexpr int c1x(int l, int f)
+{
+    int n = 0;
+    for (int lx = l; lx >= 0; lx--)
+        for (int ly = l - lx; ly >= 0; ly--) {
+            if (n == f) return lx;
+            n++;
+        }
+    return 0;
+}
+__host__ __device__ constexpr int c1y(int l, int f)
+{
+    int n = 0;
+    for (int lx = l; lx >= 0; lx--)
+        for (int ly = l - lx; ly >= 0; ly--) {
+            if (n == f) return ly;
+            n++;
+        }
+    return 0;
+}
 
 // value of the primitive integral with bra powers ia, ket powers jb
 template <int LI, int LJ>
@@ -171,11 +205,11 @@ struct DerivTables {
     double w[NR];
 };
 
-template <int LI, int LJ>
-__device__ __forceinline__ double deriv_val(const DerivTables<LI, LJ> &T, int typ, const int *ia, const int *jb)
+template <int LI, int LJ, int TYP>
+__device__ __forceinline__ double deriv_val(const DerivTables<LI, LJ> &T, const int *ia, const int *jb)
 {
-    if (typ == 0) return T.s[0][ia[0]][jb[0]] * T.s[1][ia[1]][jb[1]] * T.s[2][ia[2]][jb[2]];
-    if (typ == 1) {
+    if constexpr (TYP == 0) return T.s[0][ia[0]][jb[0]] * T.s[1][ia[1]][jb[1]] * T.s[2][ia[2]][jb[2]];
+    if constexpr (TYP == 1) {
         const double sx = T.s[0][ia[0]][jb[0]], sy = T.s[1][ia[1]][jb[1]], sz = T.s[2][ia[2]][jb[2]];
         return T.t[0][ia[0]][jb[0]] * sy * sz + sx * T.t[1][ia[1]][jb[1]] * sz + sx * sy * T.t[2][ia[2]][jb[2]];
     }
@@ -186,35 +220,40 @@ __device__ __forceinline__ double deriv_val(const DerivTables<LI, LJ> &T, int ty
     return acc;
 }
 
-// W-weighted sums of one charge (typ 2) or of S/T over the components of
-// the pair: adds exponent / coefficient derivatives of bra primitive ip and
-// the bra-centre gradient; dS mode writes the matrix entries instead.
-template <int LI, int LJ>
-__device__ __forceinline__ void deriv_accumulate(const DerivTables<LI, LJ> &T, int typ, double pref, double alpha,
-                                                 double ca, const Shell1Dev &A, const Shell1Dev &B, int ip,
+// W-weighted sums of one charge (V) or of S/T over the components of the
+// pair: adds exponent / coefficient derivatives of bra primitive ip and the
+// bra-centre gradient; dS mode writes the matrix entries instead.  The
+// component loops are unrolled up to d shells, so that the tables are
+// indexed with constants and stay in registers.
+template <int LI, int LJ, int TYP>
+__device__ __forceinline__ void deriv_accumulate(const DerivTables<LI, LJ> &T, double pref, double alpha, double ca,
+                                                 const Shell1Dev &A, const Shell1Dev &B, int ip,
                                                  const double *W, int nao, double &dex, double &dco, double gA[3],
                                                  double *dS, long long s1, long long L)
 {
     constexpr int NFI = (LI + 1) * (LI + 2) / 2, NFJ = (LJ + 1) * (LJ + 2) / 2;
+    constexpr int UNR = (LI <= 2 && LJ <= 2) ? 64 : 1;
     const size_t n = (size_t)nao;
+#pragma unroll UNR
     for (int fj = 0; fj < NFJ; fj++)
+#pragma unroll UNR
         for (int fi = 0; fi < NFI; fi++) {
-            const int ia[3] = {C1E[LI][fi][0], C1E[LI][fi][1], C1E[LI][fi][2]};
-            const int jb[3] = {C1E[LJ][fj][0], C1E[LJ][fj][1], C1E[LJ][fj][2]};
-            const double v0 = pref * deriv_val<LI, LJ>(T, typ, ia, jb);
+            const int ia[3] = {c1x(LI, fi), c1y(LI, fi), LI - c1x(LI, fi) - c1y(LI, fi)};
+            const int jb[3] = {c1x(LJ, fj), c1y(LJ, fj), LJ - c1x(LJ, fj) - c1y(LJ, fj)};
+            const double v0 = pref * deriv_val<LI, LJ, TYP>(T, ia, jb);
             double de = 0.0, dA[3];
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 int up[3] = {ia[0], ia[1], ia[2]};
                 up[d] += 2;
-                de -= pref * deriv_val<LI, LJ>(T, typ, up, jb);
+                de -= pref * deriv_val<LI, LJ, TYP>(T, up, jb);
                 int p1[3] = {ia[0], ia[1], ia[2]};
                 p1[d] += 1;
-                double g = 2.0 * alpha * pref * deriv_val<LI, LJ>(T, typ, p1, jb);
+                double g = 2.0 * alpha * pref * deriv_val<LI, LJ, TYP>(T, p1, jb);
                 if (ia[d] > 0) {
                     int m1[3] = {ia[0], ia[1], ia[2]};
                     m1[d] -= 1;
-                    g -= ia[d] * pref * deriv_val<LI, LJ>(T, typ, m1, jb);
+                    g -= ia[d] * pref * deriv_val<LI, LJ, TYP>(T, m1, jb);
                 }
                 dA[d] = g;
             }
@@ -226,7 +265,7 @@ __device__ __forceinline__ void deriv_accumulate(const DerivTables<LI, LJ> &T, i
 #pragma unroll
                 for (int d = 0; d < 3; d++) gA[d] += w2 * ca * dA[d];
             }
-            if (dS) {
+            if (TYP == 0 && dS) {
                 const long long ie = A.pexp + ip, ic = A.pcoef + ip;
                 if (ie >= s1 && ie < s1 + L) {
                     atomicAdd(&dS[(a * n + b) * L + (ie - s1)], ca * de);
@@ -240,22 +279,35 @@ __device__ __forceinline__ void deriv_accumulate(const DerivTables<LI, LJ> &T, i
         }
 }
 
-template <int LI, int LJ>
+// sum over the aligned segment of cw lanes (a power of two; every lane of
+// the segment active)
+__device__ __forceinline__ double seg_sum(double v, int cw)
+{
+    for (int off = 1; off < cw; off <<= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int LI, int LJ, int TYP>
 __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict__ pairs, int npair,
                                                          const Shell1Dev *__restrict__ sh,
                                                          const double *__restrict__ env,
-                                                         const double4 *__restrict__ nuc, int nnuc, int nao, int typ,
+                                                         const double4 *__restrict__ nuc, int nnuc, int cwlog, int nao,
                                                          const double *__restrict__ W, double *__restrict__ gatom,
                                                          double *__restrict__ denv, long long env_l,
                                                          double *__restrict__ dS, long long s1, long long L)
 {
     using TT = DerivTables<LI, LJ>;
-    // typ 2: one thread per (pair, point charge); otherwise one per pair
-    const int nsub = typ == 2 ? nnuc : 1;
-    const long long tt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tt >= (long long)npair * nsub) return;
-    const int t = (int)(tt / nsub), csel = (int)(tt - (long long)t * nsub);
-    const int2 pr = pairs[t];
+    // lanes per pair: 1 for S and T, 2^cwlog for V
+    const int cw = TYP == 2 ? 1 << cwlog : 1;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = TYP == 2 ? gid >> cwlog : gid;
+    const int cs = TYP == 2 ? (int)(gid & (cw - 1)) : 0;
+    if (TYP != 2 && t >= npair) return;
+    // V: every lane stays to the end (the wavefront-wide sum of the charge
+    // gradients); a segment past the last pair runs no primitive loop
+    const bool live = t < npair;
+    const bool one = nnuc <= cw;  // at most one charge per lane
+    const int2 pr = pairs[live ? t : 0];
     const Shell1Dev A = sh[pr.x], B = sh[pr.y];  // A.pad = atom of A
     double AB[3], AB2 = 0.0;
 #pragma unroll
@@ -265,8 +317,9 @@ __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict_
     }
     const double cf = fac_sp_dev(LI) * fac_sp_dev(LJ);
     TT T;
-    double gAt[3] = {0.0, 0.0, 0.0};
-    for (int ip = 0; ip < A.nprim; ip++) {
+    double gAt[3] = {0.0, 0.0, 0.0}, gC[3] = {0.0, 0.0, 0.0};
+    const int nip = live ? A.nprim : 0;
+    for (int ip = 0; ip < nip; ip++) {
         const double a = env[A.pexp + ip], ca = env[A.pcoef + ip];
         double dex = 0.0, dco = 0.0;
         for (int jp = 0; jp < B.nprim; jp++) {
@@ -276,7 +329,7 @@ __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict_
 #pragma unroll
             for (int d = 0; d < 3; d++) P[d] = (a * A.r[d] + b * B.r[d]) / p;
             const double pref0 = cb * cf * exp(-a * b / p * AB2);  // bra coefficient left out
-            if (typ <= 1) {
+            if constexpr (TYP <= 1) {
                 const double s00 = sqrt(M_PI / p);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
@@ -297,7 +350,7 @@ __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict_
                             }
                             T.s[d][i][j] = v;
                         }
-                    if (typ == 1) {
+                    if constexpr (TYP == 1) {
 #pragma unroll
                         for (int i = 0; i < TT::IM; i++)
 #pragma unroll
@@ -308,9 +361,9 @@ __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict_
                             }
                     }
                 }
-                deriv_accumulate<LI, LJ>(T, typ, pref0, a, ca, A, B, ip, W, nao, dex, dco, gAt, dS, s1, L);
+                deriv_accumulate<LI, LJ, TYP>(T, pref0, a, ca, A, B, ip, W, nao, dex, dco, gAt, dS, s1, L);
             } else {
-                for (int c = csel; c < csel + 1; c++) {
+                for (int c = cs; c < nnuc; c += cw) {
                     const double4 C = nuc[c];
                     const double PC[3] = {P[0] - C.x, P[1] - C.y, P[2] - C.z};
                     const double PC2 = PC[0] * PC[0] + PC[1] * PC[1] + PC[2] * PC[2];
@@ -340,23 +393,44 @@ __global__ __launch_bounds__(64) void int1e_deriv_kernel(const int2 *__restrict_
                     }
                     double gA[3] = {0.0, 0.0, 0.0};
                     const double pref = -C.w * 2.0 * M_PI / p * pref0;
-                    deriv_accumulate<LI, LJ>(T, 2, pref, a, ca, A, B, ip, W, nao, dex, dco, gA, dS, s1, L);
-                    if (gatom) {
+                    deriv_accumulate<LI, LJ, 2>(T, pref, a, ca, A, B, ip, W, nao, dex, dco, gA, nullptr, 0, 0);
 #pragma unroll
-                        for (int d = 0; d < 3; d++) {
-                            gAt[d] += gA[d];
-                            atomicAdd(&gatom[3 * c + d], -gA[d]);
-                        }
+                    for (int d = 0; d < 3; d++) {
+                        gAt[d] += gA[d];
+                        if (one) gC[d] += gA[d];
+                        else if (gatom) atomicAdd(&gatom[3 * c + d], -gA[d]);
                     }
                 }
             }
         }
-        if (denv) {
+        if (TYP == 2) {
+            // the pair's lanes run the same primitive loops: all active here
+            dex = seg_sum(dex, cw);
+            dco = seg_sum(dco, cw);
+        }
+        if (denv && cs == 0) {
             if (A.pexp + ip < env_l) atomicAdd(&denv[A.pexp + ip], dex);
             if (A.pcoef + ip < env_l) atomicAdd(&denv[A.pcoef + ip], dco);
         }
     }
-    if (gatom) {
+    if (!gatom) return;
+    if constexpr (TYP == 2) {
+        if (live) {
+#pragma unroll
+            for (int d = 0; d < 3; d++) gAt[d] = seg_sum(gAt[d], cw);
+        }
+        if (one) {
+            // every lane of the wavefront: charge cs summed over the segments
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                for (int off = cw; off < 64; off <<= 1) gC[d] += __shfl_xor(gC[d], off, 64);
+            if ((int)threadIdx.x < cw && cs < nnuc) {
+#pragma unroll
+                for (int d = 0; d < 3; d++) atomicAdd(&gatom[3 * cs + d], -gC[d]);
+            }
+        }
+    }
+    if (live && cs == 0) {
 #pragma unroll
         for (int d = 0; d < 3; d++) atomicAdd(&gatom[3 * A.pad + d], gAt[d]);
     }
@@ -369,13 +443,32 @@ hipError_t launch_int1e_deriv(int li, int lj, const int2 *pairs, int npair, cons
     if (npair <= 0) return hipSuccess;
     if (typ < 0 || typ > 2 || (dS && typ != 0)) return hipErrorInvalidValue;
     if (typ == 2 && nnuc <= 0) return hipSuccess;
-    const long long nthr = (long long)npair * (typ == 2 ? nnuc : 1);
+    // V: lanes per pair, the charge count rounded up to a power of two, at
+    // most 64 (LIBRINT_V_LANES lowers the cap: fewer lanes per pair, each
+    // taking several charges)
+    int cwmax = 6;
+    if (const char *e = getenv("LIBRINT_V_LANES")) {
+        const int v = atoi(e);
+        cwmax = 0;
+        while (cwmax < 6 && (2 << cwmax) <= v) cwmax++;
+    }
+    int cwlog = 0;
+    if (typ == 2)
+        while ((1 << cwlog) < nnuc && cwlog < cwmax) cwlog++;
+    const long long nthr = (long long)npair << (typ == 2 ? cwlog : 0);
     if ((nthr + 63) / 64 > 0x7fffffffLL) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((nthr + 63) / 64));
 #define X(a, b)                                                                                                        \
     if (li == a && lj == b) {                                                                                          \
-        hipLaunchKernelGGL((int1e_deriv_kernel<a, b>), grid, dim3(64), 0, st, pairs, npair, sh, env, nuc, nnuc, nao,   \
-                           typ, W, gatom, denv, env_l, dS, s1, L);                                                     \
+        if (typ == 0)                                                                                                  \
+            hipLaunchKernelGGL((int1e_deriv_kernel<a, b, 0>), grid, dim3(64), 0, st, pairs, npair, sh, env, nuc, nnuc, \
+                               cwlog, nao, W, gatom, denv, env_l, dS, s1, L);                                          \
+        else if (typ == 1)                                                                                             \
+            hipLaunchKernelGGL((int1e_deriv_kernel<a, b, 1>), grid, dim3(64), 0, st, pairs, npair, sh, env, nuc, nnuc, \
+                               cwlog, nao, W, gatom, denv, env_l, dS, s1, L);                                          \
+        else                                                                                                           \
+            hipLaunchKernelGGL((int1e_deriv_kernel<a, b, 2>), grid, dim3(64), 0, st, pairs, npair, sh, env, nuc, nnuc, \
+                               cwlog, nao, W, gatom, denv, env_l, dS, s1, L);                                          \
         return hipGetLastError();                                                                                      \
     }
     X(0, 0) X(0, 1) X(0, 2) X(1, 0) X(1, 1) X(1, 2) X(2, 0) X(2, 1) X(2, 2)

@@ -132,6 +132,30 @@ def test_translational_invariance(gpu):
         assert np.abs(s).max() < 1e-10 * scale, (k, s)
 
 
+@pytest.mark.parametrize("name,lanes", [("w2", "1"), ("w2", "2"), ("h2o_fg", "2")])
+def test_v_derivative_lane_shapes_agree(gpu, name, lanes, monkeypatch):
+    """The nuclear-attraction derivative kernel with fewer lanes per shell
+    pair than point charges (each lane then takes several charges and adds
+    each charge's gradient per primitive pair) gives the same one-electron
+    parts as with one charge per lane (the default up to 64 charges)."""
+    from librint_amd.context import FockContext
+    from librint_amd import autograd, dscf
+    m = _mols()[name]()
+    P = _density(m.nao(), 9)
+    ctx = FockContext(m, 0.0)
+    monkeypatch.delenv("LIBRINT_V_LANES", raising=False)
+    _, _, ref = autograd.gradient(ctx, P, parts=True)
+    dH0 = dscf.dHcoref(m, P)
+    monkeypatch.setenv("LIBRINT_V_LANES", lanes)
+    _say("V derivative lanes", name, lanes, "natm", len(m._atm))
+    _, _, got = autograd.gradient(ctx, P, parts=True)
+    dH1 = dscf.dHcoref(m, P)
+    ctx.close()
+    scale = max(1.0, np.abs(ref["gH"]).max())
+    assert np.abs(got["gH"] - ref["gH"]).max() < 1e-11 * scale
+    assert np.abs(dH1 - dH0).max() < 1e-11 * max(1.0, np.abs(dH0).max())
+
+
 def _exp_step(alpha):
     """Exponent step 1e-4*max(zeta, 1) for the 5-point stencil: re-converged
     energies carry ~5e-12 Eh of noise (FP64 atomics in the Fock build, SCF
