@@ -17,12 +17,32 @@
 
 namespace librint {
 
-__device__ __constant__ signed char C1E[5][15][3] = {
-    {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
-    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
-    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
-    {{3, 0, 0}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {1, 1, 1}, {1, 0, 2}, {0, 3, 0}, {0, 2, 1}, {0, 1, 2}, {0, 0, 3}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}},
-    {{4, 0, 0}, {3, 1, 0}, {3, 0, 1}, {2, 2, 0}, {2, 1, 1}, {2, 0, 2}, {1, 3, 0}, {1, 2, 1}, {1, 1, 2}, {1, 0, 3}, {0, 4, 0}, {0, 3, 1}, {0, 2, 2}, {0, 1, 3}, {0, 0, 4}}};
+// cartesian exponents of component f of a shell of angular momentum l, in
+// libcint's order (x exponent descending, then y descending)
+__host__ __device__ constexpr int c1x(int l, int f)
+{
+    int n = 0;
+    for (int lx = l; lx >= 0; lx--)
+        for (int ly = l - lx; ly >= 0; ly--) {
+            if (n == f) return lx;
+            n++;
+        }
+    return 0;
+}
+__host__ __device__ constexpr int c1y(int l, int f)
+{
+    int n = 0;
+    for (int lx = l; lx >= 0; lx--)
+        for (int ly = l - lx; ly >= 0; ly--) {
+            if (n == f) return ly;
+            n++;
+        }
+    return 0;
+}
+__host__ __device__ constexpr int c1e(int l, int f, int d)
+{
+    return d == 0 ? c1x(l, f) : d == 1 ? c1y(l, f) : l - c1x(l, f) - c1y(l, f);
+}
 
 __device__ __forceinline__ double fac_sp_dev(int l)
 {
@@ -92,7 +112,7 @@ __global__ __launch_bounds__(128) void int1e_kernel(const int2 *__restrict__ pai
                         double sv[3], tv[3];
 #pragma unroll
                         for (int d = 0; d < 3; d++) {
-                            const int i = C1E[LI][fi][d], j = C1E[LJ][fj][d];
+                            const int i = c1e(LI, fi, d), j = c1e(LJ, fj, d);
                             sv[d] = s[d][i][j];
                             double tt = -2.0 * b * (2 * j + 1) * s[d][i][j] + 4.0 * b * b * s[d][i][j + 2];
                             if (j >= 2) tt += j * (j - 1) * s[d][i][j - 2];
@@ -132,9 +152,9 @@ __global__ __launch_bounds__(128) void int1e_kernel(const int2 *__restrict__ pai
                         for (int fj = 0; fj < NFJ; fj++)
 #pragma unroll
                             for (int fi = 0; fi < NFI; fi++)
-                                bv[fi + NFI * fj] += fw * I[0][C1E[LI][fi][0]][C1E[LJ][fj][0]] *
-                                                     I[1][C1E[LI][fi][1]][C1E[LJ][fj][1]] *
-                                                     I[2][C1E[LI][fi][2]][C1E[LJ][fj][2]];
+                                bv[fi + NFI * fj] += fw * I[0][c1e(LI, fi, 0)][c1e(LJ, fj, 0)] *
+                                                     I[1][c1e(LI, fi, 1)][c1e(LJ, fj, 1)] *
+                                                     I[2][c1e(LI, fi, 2)][c1e(LJ, fj, 2)];
                     }
                 }
             }
@@ -171,29 +191,6 @@ __global__ __launch_bounds__(128) void int1e_kernel(const int2 *__restrict__ pai
 // charge per lane (at most 64 charges) each charge's gradient is summed over
 // the pair's primitives in registers and over the wavefront's segments
 // before one atomic per charge and wavefront.
-
-// cartesian exponents of component f of a shell of angular momentum l, in
-// the order of C1E (x descending, then y descending)
-__host__ __device__ constexpr int c1x(int l, int f)
-{
-    int n = 0;
-    for (int lx = l; lx >= 0; lx--)
-        for (int ly = l - lx; ly >= 0; ly--) {
-            if (n == f) return lx;
-            n++;
-        }
-    return 0;
-}
-__host__ __device__ constexpr int c1y(int l, int f)
-{
-    int n = 0;
-    for (int lx = l; lx >= 0; lx--)
-        for (int ly = l - lx; ly >= 0; ly--) {
-            if (n == f) return ly;
-            n++;
-        }
-    return 0;
-}
 
 // value of the primitive integral with bra powers ia, ket powers jb
 template <int LI, int LJ>
