@@ -130,11 +130,26 @@ __device__ __forceinline__ void grad_chain(double *g, int c, const PrimPair &p, 
     const double cn = sel3(ax, p.pbx, p.pby, p.pbz) - akl * tr * pq;
     const double cm = sel3(ax, q.pbx, q.pby, q.pbz) + aij * tr * pq;
     double *gc = g + c * S::GS;
-    gc[0] = (ax == 2) ? wr * fac1 : 1.0;
-    vrr_rolled<S::NMAX, S::MMAX, S::DL, S::DJ>(gc, cn, cm, b10, b01, b00);
-    if constexpr (S::LA2 > 0 || S::LC2 > 0)
-        hrr_rolled<S::LA2, S::LB2, S::LC2, S::NMAX, S::MMAX, S::DK, S::DL, S::DJ>(gc, sel3(ax, rab[0], rab[1], rab[2]),
-                                                                               sel3(ax, rcd[0], rcd[1], rcd[2]));
+    if constexpr (S::GS <= 64) {
+        // short chains: the recurrences unrolled in registers, then one
+        // store of the chain to LDS
+        double gl[S::GS];
+#pragma unroll
+        for (int e = 0; e < S::GS; e++) gl[e] = 0.0;
+        gl[0] = (ax == 2) ? wr * fac1 : 1.0;
+        vrr_axis<S::NMAX, S::MMAX, S::DL, S::DJ>(gl, cn, cm, b10, b01, b00);
+        if constexpr (S::LA2 > 0 || S::LC2 > 0)
+            hrr_axis<S::LA2, S::LB2, S::LC2, S::NMAX, S::MMAX, S::DK, S::DL, S::DJ>(
+                gl, sel3(ax, rab[0], rab[1], rab[2]), sel3(ax, rcd[0], rcd[1], rcd[2]));
+#pragma unroll
+        for (int e = 0; e < S::GS; e++) gc[e] = gl[e];
+    } else {
+        gc[0] = (ax == 2) ? wr * fac1 : 1.0;
+        vrr_rolled<S::NMAX, S::MMAX, S::DL, S::DJ>(gc, cn, cm, b10, b01, b00);
+        if constexpr (S::LA2 > 0 || S::LC2 > 0)
+            hrr_rolled<S::LA2, S::LB2, S::LC2, S::NMAX, S::MMAX, S::DK, S::DL, S::DJ>(
+                gc, sel3(ax, rab[0], rab[1], rab[2]), sel3(ax, rcd[0], rcd[1], rcd[2]));
+    }
 }
 
 template <int LA, int LB, int LC, int LD, int SIDE>
