@@ -932,9 +932,12 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
     check(hipEventRecord(ev0_, st), "hipEventRecord");
     // The launch with the most flops runs first and alone on st: it fills the
     // device by itself and its timing stays its own.  The others are then
-    // dealt round-robin over st and ns - 1 extra streams; they share nothing
-    // but M (atomics) and, in the per-launch slab mode, their stream's slab
-    // buffer.
+    // dealt round-robin over st and ns - 1 extra streams, in order of
+    // decreasing flops per task, so that the classes with long tasks and few
+    // wavefronts per SIMD start early beside the others instead of running
+    // alone at the end (measured: C16H34 1.53 -> 1.49 ms, (H2O)32 36.07 ->
+    // 35.89 ms); they share nothing but M (atomics) and, in the per-launch
+    // slab mode, their stream's slab buffer.
     const int nl = (int)launches_.size();
     // measured: two streams are best (three and four were slower)
     const int nstreams = nstreams_ > 0 ? nstreams_ : 2;
@@ -951,8 +954,15 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
     ns = std::max(ns, 1);
     if (sc) ns = 1;
     if (ns > 1) ensure_side_streams(ns - 1);
+    std::vector<int> order;
+    order.push_back(dominant_);
+    for (int k = 0; k < nl; k++)
+        if (k != dominant_) order.push_back(k);
+    std::stable_sort(order.begin() + 1, order.end(), [&](int x, int y) {
+        return launches_[x].flops / std::max(1, launches_[x].count) > launches_[y].flops / std::max(1, launches_[y].count);
+    });
     for (int j = 0; j < nl; j++) {
-        const int k = j == 0 ? dominant_ : (j <= dominant_ ? j - 1 : j);
+        const int k = order[j];
         ClassLaunch &L = launches_[k];
         if (j == 1 && ns > 1) {
             check(hipEventRecord(evf_, st), "hipEventRecord");
