@@ -337,7 +337,7 @@ def fock_line(name, steps, warmup, tau_mix_list, traffic_precision="fp64"):
                    "precision": "fp64", "quartets": int(quartets), "prim_quartets": int(st["prim_quartets"]),
                    "integrals": int(st["integrals"]), "candidates": int(st["candidates"]),
                    "launches": int(st["launches"]),
-                   "streams": int(os.environ.get("LIBRINT_STREAMS", "2"))},
+                   "streams": int(ctx.stats()["streams"])},
         "roofline": {"bound": "valu_fp64", "achieved": dom_achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": dom_achieved / FP64_PEAK_TFLOPS,
                      "traffic": TRAFFIC.get((name, traffic_precision, dom["kernel"])),

@@ -158,11 +158,12 @@ size_t librint_int2e_shells(librint_ctx *ctx, const int *quartets, int nq, doubl
  * (nbas*(nbas+1)/2 doubles). Returns 0 on success. */
 int librint_schwarz(librint_ctx *ctx, double *out);
 
-/* Counters of the current quartet plan, out[0..9]:
+/* Counters of the current quartet plan, out[0..10]:
  * quartets, primitive quartets, contracted integrals, algorithmic flops,
  * candidate quartets, kernel launches, device ms of the last build, Schwarz tol,
  * device ms of the last adjoint (gradient) kernel launches, densities the last
- * SCF on the context took from the purified projector instead of eigh.
+ * SCF on the context took from the purified projector instead of eigh,
+ * streams the last build's class launches ran on.
  * Returns the number of values written. */
 int librint_ctx_stats(const librint_ctx *ctx, double *out, int n);
 

@@ -7,7 +7,7 @@ from ._bindings import library
 from . import utils
 
 STAT_KEYS = ("quartets", "prim_quartets", "integrals", "flops", "candidates", "launches",
-             "kernel_ms", "schwarz_tol", "grad_kernel_ms", "scf_purified")
+             "kernel_ms", "schwarz_tol", "grad_kernel_ms", "scf_purified", "streams")
 
 
 class FockContext:

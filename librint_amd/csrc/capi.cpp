@@ -679,11 +679,11 @@ int librint_ctx_stats(const librint_ctx *ctx, double *out, int n)
 {
     if (!ctx || !out) return -1;
     const Stats &s = ctx->ctx->stats();
-    const double v[10] = {(double)s.quartets, (double)s.prim_quartets, (double)s.integrals, s.flops,
+    const double v[11] = {(double)s.quartets, (double)s.prim_quartets, (double)s.integrals, s.flops,
                           (double)s.candidates, (double)s.launches, (double)s.last_kernel_ms, ctx->ctx->schwarz(),
-                          (double)s.last_grad_ms, (double)s.scf_purified};
+                          (double)s.last_grad_ms, (double)s.scf_purified, (double)s.streams};
     int k = 0;
-    for (; k < n && k < 10; k++) out[k] = v[k];
+    for (; k < n && k < 11; k++) out[k] = v[k];
     return k;
 }
 

@@ -454,6 +454,14 @@ bool Context::drow_plan(int la, int lb, int lc, int ld, int fp32, double prim_qu
            (lds_drow <= (size_t)max_lds_ / 2 || jk_group_waves(la, lb, lc, ld, true) > 4);
 }
 
+// Streams for the class launches after the dominant one (LIBRINT_STREAMS
+// overrides): measured with the launches in order of decreasing flops per
+// task, three streams are best for builds of many class launches ((H2O)32,
+// 21 launches: 35.79-35.83 against 35.99-36.00 ms with two; four slower)
+// and two for a few (C16H34, 6 launches: 1.487-1.490 against 1.494-1.501 ms
+// with three).
+int Context::default_streams() const { return launches_.size() > 12 ? 3 : 2; }
+
 void Context::build_tasks()
 {
     // the lists below replace device buffers that launches of an earlier
@@ -745,7 +753,7 @@ void Context::build_tasks()
     d_item_tasks_ = dev_upload(item_tasks);
     // chunked mode: one buffer per stream, so that launches on different
     // streams never share slab memory
-    slab_streams_ = jslab_whole_ ? 1 : std::max(1, nstreams_ > 0 ? nstreams_ : 2);
+    slab_streams_ = jslab_whole_ ? 1 : std::max(1, nstreams_ > 0 ? nstreams_ : default_streams());
     jslot_ = jslab_whole_ ? total : budget;
     const size_t need = jslab_whole_ ? total : budget * slab_streams_;
     if (need > nJ_) {
@@ -939,8 +947,7 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
     // 35.89 ms); they share nothing but M (atomics) and, in the per-launch
     // slab mode, their stream's slab buffer.
     const int nl = (int)launches_.size();
-    // measured: two streams are best (three and four were slower)
-    const int nstreams = nstreams_ > 0 ? nstreams_ : 2;
+    const int nstreams = nstreams_ > 0 ? nstreams_ : default_streams();
     if (!sc) {
         std::vector<std::array<int, 4>> cl;
         for (const ClassLaunch &L : launches_) cl.push_back({L.la, L.lb, L.lc, L.ld});
@@ -954,6 +961,7 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
     ns = std::max(ns, 1);
     if (sc) ns = 1;
     if (ns > 1) ensure_side_streams(ns - 1);
+    stats_.streams = ns;
     std::vector<int> order;
     order.push_back(dominant_);
     for (int k = 0; k < nl; k++)

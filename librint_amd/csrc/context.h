@@ -33,6 +33,7 @@ struct Stats {
     float last_kernel_ms = 0.f;    // device time of the last build (events)
     float last_grad_ms = 0.f;      // device time of the last adjoint (grad2e) launches
     int scf_purified = 0;          // densities by purification in the last SCF on the context
+    int streams = 0;               // streams of the last build's class launches
 };
 
 struct ClassLaunch {
@@ -121,6 +122,7 @@ private:
     void build_pairs();
     void compute_schwarz();
     void build_tasks();
+    int default_streams() const;
     bool lane_path(int la, int lb, int lc, int ld, int mixed) const;
     bool drow_plan(int la, int lb, int lc, int ld, int fp32, double prim_quartets, double quartets) const;
     void upload();
@@ -235,7 +237,7 @@ private:
     // caller's stream and nstreams_ - 1 extra streams (LIBRINT_STREAMS),
     // joined before the Coulomb reduction: small launches and launch tails
     // overlap
-    int nstreams_ = 0;      // 0: 2
+    int nstreams_ = 0;      // 0: default_streams()
     int team_kets_ = 8;     // ket window of the tasks of the team and f/g classes (LIBRINT_TEAM_KETS)
     // lane-per-quartet classes: a task whose lanes would each evaluate more
     // than prim_split_ primitive quartets is split by bra primitive pairs
