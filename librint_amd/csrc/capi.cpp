@@ -237,11 +237,15 @@ GradParts grad_parts(Context *ctx, const Int1eInput &one, const double *P, size_
     const size_t na = one.charges.size();
     const int n = one.nao;
     const size_t nn = (size_t)n * n;
+    // the one-electron derivative passes run together at the end, with one
+    // staging of the basis on the GPU
+    std::vector<Int1eGradPass> passes;
+    std::vector<double> Q;
     if (what & GP_H) {
         g.dH.assign(env_l, 0.0);
         g.gH.assign(3 * na, 0.0);
-        int1e_grad(one, 1, P, g.gH.data(), g.dH.data(), env_l);
-        int1e_grad(one, 2, P, g.gH.data(), g.dH.data(), env_l);
+        passes.push_back(Int1eGradPass{1, P, g.gH.data(), g.dH.data()});
+        passes.push_back(Int1eGradPass{2, P, g.gH.data(), g.dH.data()});
     }
     if (what & GP_R) {
         g.dR.assign(env_l, 0.0);
@@ -250,7 +254,8 @@ GradParts grad_parts(Context *ctx, const Int1eInput &one, const double *P, size_
     }
     if (what & GP_S) {
         // Q = P F P with F = T + V + G[P]
-        std::vector<double> F(nn), T(nn), G(nn), Q(nn);
+        std::vector<double> F(nn), T(nn), G(nn);
+        Q.assign(nn, 0.0);
         int1e_gpu(one, nullptr, T.data(), F.data());
         ctx->fock_host(P, G.data());
         for (size_t k = 0; k < nn; k++) F[k] += T[k] + G[k];
@@ -263,8 +268,9 @@ GradParts grad_parts(Context *ctx, const Int1eInput &one, const double *P, size_
             }
         g.dS.assign(env_l, 0.0);
         g.gS.assign(3 * na, 0.0);
-        int1e_grad(one, 0, Q.data(), g.gS.data(), g.dS.data(), env_l);
+        passes.push_back(Int1eGradPass{0, Q.data(), g.gS.data(), g.dS.data()});
     }
+    int1e_grad_passes(one, passes, env_l);
     g.gN.assign(3 * na, 0.0);
     nuclear_repulsion_grad(one, g.gN.data());
     return g;

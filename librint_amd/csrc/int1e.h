@@ -31,6 +31,13 @@ double nuclear_repulsion(const Int1eInput &in);
 // derivative with respect to the point-charge positions).  Either output may
 // be null.
 void int1e_grad(const Int1eInput &in, int typ, const double *W, double *gatom, double *denv, size_t env_l);
+// Several such passes with one staging of the basis on the GPU.
+struct Int1eGradPass {
+    int typ;
+    const double *W;
+    double *gatom, *denv;
+};
+void int1e_grad_passes(const Int1eInput &in, const std::vector<Int1eGradPass> &passes, size_t env_l);
 // d/dO_ab of every overlap element: dS[(a*nao + b)*L + (e - s1)] for env
 // entries s1 <= e < s1 + L.
 void overlap_param_derivs(const Int1eInput &in, size_t s1, size_t L, double *dS);

@@ -3,6 +3,7 @@
 // and copies the matrices (or the derivative sums) back.
 #include "int1e_gpu.h"
 
+#include <memory>
 #include <stdexcept>
 #include <string>
 
@@ -87,10 +88,13 @@ void int1e_gpu_device(const Int1eInput &in, double *dS, double *dT, double *dV, 
     ck(hipStreamSynchronize(st), "hipStreamSynchronize");
 }
 
-// One derivative pass on the GPU: W-weighted sums into gatom/denv (added to
-// the host arrays), or the dS matrix (typ 0, W null).
-static void deriv_gpu(const Int1eInput &in, int typ, const double *W, double *gatom, double *denv, size_t env_l,
-                      double *dSout, size_t s1, size_t L)
+// Derivative passes on the GPU with one staging of the shells, env and
+// point charges: each pass adds its W-weighted sums into gatom/denv (host
+// arrays; passes with the same pair of targets share one device
+// accumulator, copied back and added once), or writes the dS matrix (typ 0,
+// W null).
+static void deriv_gpu(const Int1eInput &in, const std::vector<Int1eGradPass> &passes, size_t env_l, double *dSout,
+                      size_t s1, size_t L)
 {
     const Staged g = stage(in, 4, "one-electron derivatives cover l <= 4");
     const size_t nn = (size_t)in.nao * in.nao;
@@ -99,40 +103,69 @@ static void deriv_gpu(const Int1eInput &in, int typ, const double *W, double *ga
     DevBuf<double> denv_in(in.env);
     DevBuf<double4> dnuc(g.nuc);
     DevBuf<int2> dpairs(g.pairs);
-    DevBuf<double> dW(W ? nn : 0), dga(gatom ? 3 * na : 0), dde(denv ? env_l : 0), ddS(dSout ? nn * L : 0);
-    if (W) ck(hipMemcpy(dW.p, W, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
-    if (gatom) ck(hipMemset(dga.p, 0, sizeof(double) * 3 * na), "hipMemset");
-    if (denv && env_l) ck(hipMemset(dde.p, 0, sizeof(double) * env_l), "hipMemset");
+    bool anyW = false;
+    for (const Int1eGradPass &q : passes) anyW = anyW || q.W;
+    DevBuf<double> dW(anyW ? nn : 0), ddS(dSout ? nn * L : 0);
     if (dSout && nn * L) ck(hipMemset(ddS.p, 0, sizeof(double) * nn * L), "hipMemset");
-    for (int li = 0; li < 5; li++)
-        for (int lj = 0; lj < 5; lj++)
-            ck(launch_int1e_deriv(li, lj, dpairs.p + g.first[li][lj], g.count[li][lj], dsh.p, denv_in.p, dnuc.p,
-                                  (int)g.nuc.size(), in.nao, typ, dW.p, dga.p, dde.p, (long long)env_l, ddS.p,
-                                  (long long)s1, (long long)L, 0),
-               "one-electron derivative launch");
-    if (gatom) {
-        std::vector<double> h(3 * na);
-        ck(hipMemcpy(h.data(), dga.p, sizeof(double) * 3 * na, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        for (size_t k = 0; k < 3 * na; k++) gatom[k] += h[k];
+    // one device accumulator per distinct (gatom, denv) target pair
+    struct Acc {
+        double *gatom, *denv;
+        std::unique_ptr<DevBuf<double>> dga, dde;
+    };
+    std::vector<Acc> acc;
+    auto acc_of = [&](double *gatom, double *denv) -> Acc & {
+        for (Acc &a : acc)
+            if (a.gatom == gatom && a.denv == denv) return a;
+        acc.push_back(Acc{gatom, denv, std::make_unique<DevBuf<double>>(gatom ? 3 * na : 0),
+                          std::make_unique<DevBuf<double>>(denv ? env_l : 0)});
+        Acc &a = acc.back();
+        if (gatom && na) ck(hipMemset(a.dga->p, 0, sizeof(double) * 3 * na), "hipMemset");
+        if (denv && env_l) ck(hipMemset(a.dde->p, 0, sizeof(double) * env_l), "hipMemset");
+        return a;
+    };
+    for (const Int1eGradPass &q : passes) {
+        // the null stream orders this copy after the previous pass's kernels
+        if (q.W) ck(hipMemcpy(dW.p, q.W, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        Acc &a = acc_of(q.gatom, q.denv);
+        for (int li = 0; li < 5; li++)
+            for (int lj = 0; lj < 5; lj++)
+                ck(launch_int1e_deriv(li, lj, dpairs.p + g.first[li][lj], g.count[li][lj], dsh.p, denv_in.p, dnuc.p,
+                                      (int)g.nuc.size(), in.nao, q.typ, q.W ? dW.p : nullptr, a.dga->p, a.dde->p,
+                                      (long long)env_l, q.W ? nullptr : ddS.p, (long long)s1, (long long)L, 0),
+                   "one-electron derivative launch");
     }
-    if (denv && env_l) {
-        std::vector<double> h(env_l);
-        ck(hipMemcpy(h.data(), dde.p, sizeof(double) * env_l, hipMemcpyDeviceToHost), "hipMemcpy D2H");
-        for (size_t k = 0; k < env_l; k++) denv[k] += h[k];
+    for (Acc &a : acc) {
+        if (a.gatom && na) {
+            std::vector<double> h(3 * na);
+            ck(hipMemcpy(h.data(), a.dga->p, sizeof(double) * 3 * na, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            for (size_t k = 0; k < 3 * na; k++) a.gatom[k] += h[k];
+        }
+        if (a.denv && env_l) {
+            std::vector<double> h(env_l);
+            ck(hipMemcpy(h.data(), a.dde->p, sizeof(double) * env_l, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+            for (size_t k = 0; k < env_l; k++) a.denv[k] += h[k];
+        }
     }
     if (dSout && nn * L) ck(hipMemcpy(dSout, ddS.p, sizeof(double) * nn * L, hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
 
+void int1e_grad_passes(const Int1eInput &in, const std::vector<Int1eGradPass> &passes, size_t env_l)
+{
+    for (const Int1eGradPass &q : passes) {
+        if (q.typ < 0 || q.typ > 2) throw std::invalid_argument("one-electron derivative type must be 0, 1 or 2");
+        if (!q.W) throw std::invalid_argument("null weight matrix");
+    }
+    if (!passes.empty()) deriv_gpu(in, passes, env_l, nullptr, 0, 0);
+}
+
 void int1e_grad(const Int1eInput &in, int typ, const double *W, double *gatom, double *denv, size_t env_l)
 {
-    if (typ < 0 || typ > 2) throw std::invalid_argument("one-electron derivative type must be 0, 1 or 2");
-    if (!W) throw std::invalid_argument("null weight matrix");
-    deriv_gpu(in, typ, W, gatom, denv, env_l, nullptr, 0, 0);
+    int1e_grad_passes(in, {Int1eGradPass{typ, W, gatom, denv}}, env_l);
 }
 
 void overlap_param_derivs(const Int1eInput &in, size_t s1, size_t L, double *dS)
 {
-    deriv_gpu(in, 0, nullptr, nullptr, nullptr, 0, dS, s1, L);
+    deriv_gpu(in, {Int1eGradPass{0, nullptr, nullptr, nullptr}}, 0, dS, s1, L);
 }
 
 void int1e_gpu(const Int1eInput &in, double *S, double *T, double *V)

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
 
@@ -248,10 +249,23 @@ bool solve_linear(int n, std::vector<double> A, std::vector<double> b, std::vect
 
 }  // namespace
 
+// rocBLAS handle for the products outside the SCF loop: one per host thread
+// and device, created on first use and kept (never destroyed) until the
+// process ends, so that repeated calls do not pay for creating it
+static Blas &thread_blas()
+{
+    thread_local std::map<int, Blas *> handles;
+    int dev = 0;
+    hck(hipGetDevice(&dev), "hipGetDevice");
+    Blas *&b = handles[dev];
+    if (!b) b = new Blas();
+    return *b;
+}
+
 void pfp_gpu(int n, const double *P, const double *F, double *Q)
 {
     const size_t nn = (size_t)n * n;
-    Blas B;
+    Blas &B = thread_blas();
     Dev dP(nn), dF(nn), dT(nn), dQ(nn);
     hck(hipMemcpy(dP.p, P, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
     hck(hipMemcpy(dF.p, F, sizeof(double) * nn, hipMemcpyHostToDevice), "hipMemcpy H2D");
