@@ -136,18 +136,19 @@ __global__ void antisym_kernel(int n, const double *__restrict__ A, double *__re
     e[i] = A[i] - A[(size_t)r * n + c];
 }
 
-// Gershgorin bounds of a symmetric n x n matrix (one workgroup): out[0] =
+// Gershgorin bounds of an n x n matrix (one workgroup): out[0] =
 // min_j (A_jj - r_j), out[1] = max_j (A_jj + r_j), r_j the off-diagonal
-// absolute column sum.
+// absolute sum of row j (the row discs contain the spectrum as the column
+// discs do).  Thread j walks row j of the column-major matrix, so the
+// lanes of a wavefront read consecutive elements of each column.
 __global__ __launch_bounds__(1024) void gersh_kernel(int n, const double *__restrict__ A, double *__restrict__ out)
 {
     __shared__ double lo[1024], hi[1024];
     double l = 1e300, h = -1e300;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
-        const double *c = A + (size_t)j * n;
         double r = 0.0;
-        for (int i = 0; i < n; i++) r += fabs(c[i]);
-        const double d = c[j];
+        for (int i = 0; i < n; i++) r += fabs(A[(size_t)i * n + j]);
+        const double d = A[(size_t)j * n + j];
         r -= fabs(d);
         l = fmin(l, d - r);
         h = fmax(h, d + r);
