@@ -180,10 +180,28 @@ __global__ void sp2_init_kernel(int n, const double *__restrict__ A, double emin
     X[k] = ((i == j ? emax : 0.0) - A[k]) / (emax - emin);
 }
 
-// out[0] = tr X, out[1] = tr X2 (one workgroup)
-__global__ __launch_bounds__(1024) void trace2_kernel(int n, const double *__restrict__ X, const double *__restrict__ X2,
-                                                      double *__restrict__ out)
+// State of the purification on the device: the idempotency errors of the
+// last two steps, the trace at the stop, the steps taken, the last step's
+// choice and whether it has stopped.
+struct Sp2State {
+    double hist0, hist1;
+    double tr;
+    int steps, choice, done, pad;
+};
+
+// One purification step's decision (one workgroup), nothing once stopped:
+// t1 = tr X, t2 = tr X2 (X2 = X^2); stop when the idempotency error t1 - t2
+// has reached rounding level, where it no longer falls over two steps (the
+// stopping rule of Kruchinina, Rudberg and Rubensson, J. Chem. Theory
+// Comput. 12, 5788 (2016)), and is below idem_floor, so that every
+// eigenvalue of the projector is within about 1e-11 of 0 or 1 (a looser stop
+// left the SCF density unable to settle below |dP| ~ 1e-9); else choose X^2
+// (0) or 2X - X^2 (1), whichever brings the trace nearer to nocc.
+__global__ __launch_bounds__(1024) void sp2_step_kernel(int n, const double *__restrict__ X,
+                                                        const double *__restrict__ X2, Sp2State *st, double nocc,
+                                                        double idem_floor)
 {
+    if (st->done) return;
     __shared__ double a[1024], b[1024];
     double s1 = 0.0, s2 = 0.0;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -201,16 +219,26 @@ __global__ __launch_bounds__(1024) void trace2_kernel(int n, const double *__res
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = a[0];
-        out[1] = b[0];
+        const double t1 = a[0], t2 = b[0], idem = t1 - t2;
+        if ((st->steps >= 2 && idem < idem_floor && idem >= st->hist1) || idem <= 0.0) {
+            st->done = 1;
+            st->tr = t1;
+        } else {
+            st->hist1 = st->hist0;
+            st->hist0 = idem;
+            st->choice = fabs(t2 - nocc) < fabs(2.0 * t1 - t2 - nocc) ? 0 : 1;
+            st->steps++;
+        }
     }
 }
 
-// X = 2 X - X2
-__global__ void sp2_up_kernel(size_t m, double *__restrict__ X, const double *__restrict__ X2)
+// X = X2 (choice 0) or 2 X - X2 (choice 1), nothing once stopped
+__global__ void sp2_apply_kernel(size_t m, double *__restrict__ X, const double *__restrict__ X2,
+                                 const Sp2State *st)
 {
+    if (st->done) return;
     const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < m) X[k] = 2.0 * X[k] - X2[k];
+    if (k < m) X[k] = st->choice == 0 ? X2[k] : 2.0 * X[k] - X2[k];
 }
 
 __global__ void identity_kernel(int n, double *__restrict__ A)
@@ -329,7 +357,12 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
     // tridiagonal reduction.  Returns false (the caller diagonalises) when it
     // does not settle on a projector of trace nocc: a closed or vanishing
     // gap, where the aufbau occupation is not a projector's limit.
-    Dev SPa(nn), SPb(nn), sc2(2);
+    // The steps' decisions run on the device (sp2_step_kernel); the host
+    // reads the state back every fourth step only, so a step costs its
+    // product and two small kernels instead of a round trip, and at most
+    // three products run past the stop (they change nothing).
+    Dev SPa(nn), SPb(nn), sc2(2), sp2s((sizeof(Sp2State) + sizeof(double) - 1) / sizeof(double));
+    Sp2State *dst = (Sp2State *)sp2s.p;
     auto purify = [&](const double *Fo, double *Proj) -> bool {
         double hb[2];
         hipLaunchKernelGGL(gersh_kernel, dim3(1), dim3(1024), 0, 0, n, Fo, sc2.p);
@@ -338,30 +371,20 @@ ScfOut scf_run(Context &ctx, const Int1eInput &in, int nelec, int imax, double c
         if (!(hb[1] > hb[0])) return false;
         hipLaunchKernelGGL(sp2_init_kernel, dim3(blocks(nn)), dim3(256), 0, 0, n, Fo, hb[0], hb[1], Proj);
         hck(hipGetLastError(), "sp2 init launch");
-        double idem_hist[2] = {1e300, 1e300};
+        const Sp2State init{1e300, 1e300, 0.0, 0, 0, 0, 0};
+        hck(hipMemcpy(dst, &init, sizeof(init), hipMemcpyHostToDevice), "hipMemcpy H2D");
         const double idem_floor = std::max(1e-11, 1e-14 * n);
         for (int k = 0; k < 120; k++) {
             B.gemm(n, false, false, 1.0, Proj, Proj, 0.0, SPb.p);  // X^2
-            hipLaunchKernelGGL(trace2_kernel, dim3(1), dim3(1024), 0, 0, n, Proj, SPb.p, sc2.p);
-            hck(hipGetLastError(), "trace launch");
-            double t[2];
-            hck(hipMemcpy(t, sc2.p, sizeof(t), hipMemcpyDeviceToHost), "hipMemcpy D2H");
-            const double idem = t[0] - t[1];
-            // converged: the error has reached rounding level, where it no
-            // longer falls over two steps (the stopping rule of Kruchinina,
-            // Rudberg and Rubensson, J. Chem. Theory Comput. 12, 5788 (2016)),
-            // and is below idem_floor, so that every eigenvalue of the
-            // projector is within about 1e-11 of 0 or 1 (a looser stop left
-            // the SCF density unable to settle below |dP| ~ 1e-9)
-            if ((k >= 2 && idem < idem_floor && idem >= idem_hist[1]) || idem <= 0.0)
-                return std::fabs(t[0] - nocc) < 1e-6;
-            idem_hist[1] = idem_hist[0];
-            idem_hist[0] = idem;
-            if (std::fabs(t[1] - nocc) < std::fabs(2.0 * t[0] - t[1] - nocc)) {
-                B.copy(nn, SPb.p, Proj);
-            } else {
-                hipLaunchKernelGGL(sp2_up_kernel, dim3(blocks(nn)), dim3(256), 0, 0, nn, Proj, SPb.p);
-                hck(hipGetLastError(), "sp2 update launch");
+            hipLaunchKernelGGL(sp2_step_kernel, dim3(1), dim3(1024), 0, 0, n, Proj, SPb.p, dst, (double)nocc,
+                               idem_floor);
+            hck(hipGetLastError(), "sp2 step launch");
+            hipLaunchKernelGGL(sp2_apply_kernel, dim3(blocks(nn)), dim3(256), 0, 0, nn, Proj, SPb.p, dst);
+            hck(hipGetLastError(), "sp2 update launch");
+            if ((k & 3) == 3 || k == 119) {
+                Sp2State h;
+                hck(hipMemcpy(&h, dst, sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+                if (h.done) return std::fabs(h.tr - nocc) < 1e-6;
             }
         }
         return false;
