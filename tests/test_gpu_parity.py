@@ -320,23 +320,32 @@ def test_mixed_precision_fock(gpu):
 def test_stream_counts_agree(gpu, builder, monkeypatch):
     """The class launches after the first are dealt over several streams;
     any stream count gives the same G up to the summation order of the
-    atomics (FP64 and mixed lists, slab and team classes)."""
+    atomics (FP64 and mixed lists, slab and team classes).  Without
+    LIBRINT_STREAMS a build of more than 12 class launches takes three
+    streams, a smaller one two."""
     from librint_amd import mol
     from librint_amd.context import FockContext
     _say("stream counts", builder)
     m = {"c16h34": mol.c16h34_631g, "w2": lambda: mol.water_cluster(2)}[builder]()
     P = _core_density(m)
     out = {}
-    for ns in ("1", "4"):
-        monkeypatch.setenv("LIBRINT_STREAMS", ns)
+    for ns in ("1", "4", "default"):
+        if ns == "default":
+            monkeypatch.delenv("LIBRINT_STREAMS", raising=False)
+        else:
+            monkeypatch.setenv("LIBRINT_STREAMS", ns)
         ctx = FockContext(m, schwarz_tol=1e-12)
         out[ns] = ctx.fock(P)
+        st = ctx.stats()
+        want = min(int(ns), st["launches"] - 1) if ns != "default" else (3 if st["launches"] > 12 else 2)
+        assert st["streams"] == max(1, want), (ns, st["streams"], st["launches"])
         ctx.set_precision(1, 1e-5)
         out[ns + "m"] = ctx.fock(P)
         ctx.close()
     scale = np.abs(out["1"]).max()
-    assert np.abs(out["1"] - out["4"]).max() < 1e-13 * scale
-    assert np.abs(out["1m"] - out["4m"]).max() < 1e-13 * scale
+    for ns in ("4", "default"):
+        assert np.abs(out["1"] - out[ns]).max() < 1e-13 * scale
+        assert np.abs(out["1m"] - out[ns + "m"]).max() < 1e-13 * scale
     assert np.abs(out["1m"] - out["1"]).max() < 1e-6
 
 
