@@ -675,6 +675,14 @@ void Context::build_tasks()
     dominant_ = 0;
     for (size_t k = 1; k < launches_.size(); k++)
         if (launches_[k].flops > launches_[dominant_].flops) dominant_ = (int)k;
+    // launch order of a build: the dominant launch, then the others by
+    // decreasing flops per task (fock_impl)
+    launch_order_.assign(1, dominant_);
+    for (int k = 0; k < (int)launches_.size(); k++)
+        if (k != dominant_) launch_order_.push_back(k);
+    std::stable_sort(launch_order_.begin() + 1, launch_order_.end(), [&](int x, int y) {
+        return launches_[x].flops / std::max(1, launches_[x].count) > launches_[y].flops / std::max(1, launches_[y].count);
+    });
 
     // Coulomb slabs: every launch of a register class stores its ket-side
     // Coulomb blocks per task (64 NFC NFD doubles) and a reduction sums the
@@ -962,15 +970,8 @@ void Context::fock_impl(const double *D_dev, double *G_dev, hipStream_t st, cons
     if (sc) ns = 1;
     if (ns > 1) ensure_side_streams(ns - 1);
     stats_.streams = ns;
-    std::vector<int> order;
-    order.push_back(dominant_);
-    for (int k = 0; k < nl; k++)
-        if (k != dominant_) order.push_back(k);
-    std::stable_sort(order.begin() + 1, order.end(), [&](int x, int y) {
-        return launches_[x].flops / std::max(1, launches_[x].count) > launches_[y].flops / std::max(1, launches_[y].count);
-    });
     for (int j = 0; j < nl; j++) {
-        const int k = order[j];
+        const int k = launch_order_[j];
         ClassLaunch &L = launches_[k];
         if (j == 1 && ns > 1) {
             check(hipEventRecord(evf_, st), "hipEventRecord");

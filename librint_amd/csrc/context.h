@@ -246,6 +246,7 @@ private:
     // launch as one wavefront's long tail
     int prim_split_ = 48;
     int dominant_ = 0;      // launch with the most flops: runs first, alone
+    std::vector<int> launch_order_;  // dominant_, then the others by decreasing flops per task
     int slab_streams_ = 1;  // per-launch slab mode: buffers of jslot_ doubles, one per stream
     size_t jslot_ = 0;
     std::vector<hipStream_t> side_;
