@@ -688,7 +688,8 @@ void Context::build_tasks()
     // Coulomb blocks per task (64 NFC NFD doubles) and a reduction sums the
     // slabs of the tasks that share a 64-ket window.  The slab memory is
     // bounded: with S doubles of slabs in all, a build keeps them all when
-    // S <= B (one reduction after every launch), else each launch runs in
+    // S <= B (one reduction once every launch has run: measured faster on
+    // C16H34 than a reduction per launch on its stream), else each launch runs in
     // chunks of consecutive groups whose slabs fit B, each chunk reduced
     // right after it on its stream, with one B-sized buffer per stream.
     // B = LIBRINT_SLAB_MB MiB (default 1024), so the slabs take at most
