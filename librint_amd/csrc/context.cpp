@@ -115,6 +115,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     check(hipEventCreate(&ev1_), "hipEventCreate");
     if (const char *e = getenv("LIBRINT_STREAMS")) nstreams_ = std::max(1, std::min(8, atoi(e)));
     if (const char *e = getenv("LIBRINT_TEAM_KETS")) team_kets_ = std::max(1, std::min(64, atoi(e)));
+    if (const char *e = getenv("LIBRINT_GRAD_KETS")) grad_kets_ = std::max(1, std::min(64, atoi(e)));
     if (const char *e = getenv("LIBRINT_PRIM_SPLIT")) prim_split_ = std::max(0, atoi(e));
     {
         int dev = 0;
@@ -692,11 +693,13 @@ void Context::build_tasks()
     // C16H34 than a reduction per launch on its stream), else each launch runs in
     // chunks of consecutive groups whose slabs fit B, each chunk reduced
     // right after it on its stream, with one B-sized buffer per stream.
-    // B = LIBRINT_SLAB_MB MiB (default 1024), so the slabs take at most
+    // B = LIBRINT_SLAB_MB MiB (default 2304: (H2O)32's largest launch, 2153
+    // MiB of slabs, then runs as one chunk; measured with three streams
+    // 35.47-35.50 ms against 35.76-35.85 with 1024), so the slabs take at most
     // max(min(S, B), B x streams) doubles.
     std::vector<JRedItem> items;
     std::vector<int> item_tasks;
-    size_t budget = (size_t)1 << 27;  // 1 GiB of doubles
+    size_t budget = (size_t)2304 << 17;  // 2304 MiB of doubles
     if (const char *e = getenv("LIBRINT_SLAB_MB")) budget = std::max<size_t>(1, (size_t)atoll(e)) << 17;
     size_t total = 0;
     max_nfcd_ = 1;
@@ -1271,7 +1274,7 @@ void Context::build_grad_tasks()
 {
     // Ket-major list of the quartets of build_tasks (bins X >= Y, Q_i Q_j >
     // tau, j <= i inside one bin): for each ket pair j of bin Y the bras of
-    // bin X form a prefix in Q order, cut into windows of 64 (team_kets_ for
+    // bin X form a prefix in Q order, cut into windows of 64 (grad_kets_ for
     // the classes whose Fock tasks take short windows).
     glaunch1_.clear();
     if (d_gtasks1_) check(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -1295,7 +1298,7 @@ void Context::build_grad_tasks()
                 int la, lb, lc, ld;
                 cls_ls(bx.cls, la, lb);
                 cls_ls(by.cls, lc, ld);
-                const int win = class_has_fp32(la, lb, lc, ld) && lane_path(la, lb, lc, ld, 0) ? 64 : team_kets_;
+                const int win = class_has_fp32(la, lb, lc, ld) && lane_path(la, lb, lc, ld, 0) ? 64 : grad_kets_;
                 for (int i0 = X == Y ? j : 0; i0 < lo; i0 += win)
                     v.push_back(JKTask{k, bx.first + i0, std::min(win, lo - i0), 0});
             }
@@ -1319,14 +1322,36 @@ void Context::build_grad_tasks()
     glaunch0_.clear();
     dev_free(d_gtasks0_);
     std::vector<JKTask> all0;
+    std::vector<int> bin_of(pairs_.size(), -1);
+    for (int b = 0; b < nb; b++)
+        for (int k = bins_[b].first; k < bins_[b].first + bins_[b].count; k++) bin_of[k] = b;
     for (const ClassLaunch &L : launches_) {
         GradLaunch G{L.la, L.lb, L.lc, L.ld, (int)all0.size(), 0};
-        if (class_split(L.la, L.lb, L.lc, L.ld) && lane_path(L.la, L.lb, L.lc, L.ld, L.fp32))
+        const bool lane = class_has_fp32(L.la, L.lb, L.lc, L.ld) && lane_path(L.la, L.lb, L.lc, L.ld, L.fp32);
+        if (lane && class_split(L.la, L.lb, L.lc, L.ld)) {
             for (int t = L.first; t < L.first + L.count; t++) {
                 const JKTask &T = h_tasks_[t];
-                for (int k0 = 0; k0 < T.nket; k0 += team_kets_)
-                    all0.push_back(JKTask{T.bra, T.ket0 + k0, std::min(team_kets_, T.nket - k0), T.prims});
+                for (int k0 = 0; k0 < T.nket; k0 += grad_kets_)
+                    all0.push_back(JKTask{T.bra, T.ket0 + k0, std::min(grad_kets_, T.nket - k0), T.prims});
             }
+        } else if (!lane && grad_kets_ != team_kets_) {
+            // the classes whose Fock tasks take team_kets_ windows: windows
+            // of grad_kets_, cut from those tasks and joined where they
+            // continue one another over the kets of one bin (a task never
+            // spans two bins: the kernel's primitive loops are uniform)
+            for (int t = L.first; t < L.first + L.count; t++) {
+                const JKTask &T = h_tasks_[t];
+                for (int k0 = 0; k0 < T.nket; k0 += grad_kets_) {
+                    const JKTask w{T.bra, T.ket0 + k0, std::min(grad_kets_, T.nket - k0), T.prims};
+                    JKTask *last = (int)all0.size() > G.first ? &all0.back() : nullptr;
+                    if (last && last->bra == w.bra && last->prims == w.prims && last->ket0 + last->nket == w.ket0 &&
+                        bin_of[last->ket0] == bin_of[w.ket0] && last->nket + w.nket <= grad_kets_)
+                        last->nket += w.nket;
+                    else
+                        all0.push_back(w);
+                }
+            }
+        }
         G.count = (int)all0.size() - G.first;
         glaunch0_.push_back(G);
     }

@@ -213,7 +213,7 @@ private:
     std::vector<GradLaunch> glaunch1_;
     JKTask *d_gtasks1_ = nullptr;
     // bra side of the chunked classes: their Fock tasks (64-ket windows) cut
-    // into team_kets_ windows, since the adjoint kernel takes a task's kets
+    // into grad_kets_ windows, since the adjoint kernel takes a task's kets
     // one at a time (count 0: the class uses its Fock tasks as they are)
     std::vector<GradLaunch> glaunch0_;
     JKTask *d_gtasks0_ = nullptr;
@@ -238,7 +238,15 @@ private:
     // joined before the Coulomb reduction: small launches and launch tails
     // overlap
     int nstreams_ = 0;      // 0: default_streams()
-    int team_kets_ = 8;     // ket window of the tasks of the team and f/g classes (LIBRINT_TEAM_KETS)
+    // ket window of the Fock tasks of the team and f/g classes
+    // (LIBRINT_TEAM_KETS; measured on (H2O)32 with three streams: 4 gives
+    // 35.39-35.58 ms, 8 35.75-35.85, 2 35.91, 16 35.62)
+    int team_kets_ = 4;
+    // ket window of the adjoint's short tasks: the ket-major lists of those
+    // classes and the chunked classes' Fock tasks cut for the bra side
+    // (LIBRINT_GRAD_KETS; 4 instead of 8 measured 19.95-20.11 against
+    // 19.82-19.89 ms per (H2O)8 gradient)
+    int grad_kets_ = 8;
     // lane-per-quartet classes: a task whose lanes would each evaluate more
     // than prim_split_ primitive quartets is split by bra primitive pairs
     // (LIBRINT_PRIM_SPLIT, default 48; 0 = never), so that the heaviest
