@@ -116,6 +116,7 @@ Context::Context(const int *atm, size_t atm_l, const int *bas, size_t bas_l, con
     if (const char *e = getenv("LIBRINT_STREAMS")) nstreams_ = std::max(1, std::min(8, atoi(e)));
     if (const char *e = getenv("LIBRINT_TEAM_KETS")) team_kets_ = std::max(1, std::min(64, atoi(e)));
     if (const char *e = getenv("LIBRINT_GRAD_KETS")) grad_kets_ = std::max(1, std::min(64, atoi(e)));
+    if (const char *e = getenv("LIBRINT_GROUP_MIN")) group_min_ = std::max(1, atoi(e));
     if (const char *e = getenv("LIBRINT_PRIM_SPLIT")) prim_split_ = std::max(0, atoi(e));
     {
         int dev = 0;
@@ -557,8 +558,9 @@ void Context::build_tasks()
         // Group size: a group's exchange rows ((NFA+NFB) x nao in LDS) are
         // flushed with global atomics once per group, so a group should hold
         // about (NFA+NFB)*nao/64 tasks for the flush to stay small next to its
-        // quartets; at most 256, and small enough to leave >= 4096 groups per
-        // launch for load balance (never below 8).
+        // quartets; at most 64 x the workgroup's wavefronts, and small enough
+        // to leave at least group_min_ groups per launch for load balance
+        // (never below 8).
         int gla, glb, glc, gld;
         cls_ls(std::get<0>(kv.first), gla, glb);
         cls_ls(std::get<1>(kv.first), glc, gld);
@@ -580,7 +582,7 @@ void Context::build_tasks()
             // density rows in LDS and up to 768-task groups 36.35 -> 36.03 ms)
             const bool cheap = (double)A.stat.prim_quartets < 4.0 * (double)A.stat.quartets;
             cap = std::min(gmax, std::max(8, (rows * nao_ + 63) / 64 * (cheap ? (gdrow ? 12 : 4) : 1)));
-            cap = std::max(8, std::min(cap, (int)(A.tasks.size() / 4096)));
+            cap = std::max(8, std::min(cap, (int)(A.tasks.size() / group_min_)));
             cap = std::min(cap, gmax);
         }
         for (size_t t = 0; t < A.tasks.size();) {

@@ -247,6 +247,13 @@ private:
     // (LIBRINT_GRAD_KETS; 4 instead of 8 measured 19.95-20.11 against
     // 19.82-19.89 ms per (H2O)8 gradient)
     int grad_kets_ = 8;
+    // groups per launch the group size leaves at least (LIBRINT_GROUP_MIN):
+    // small groups balance a launch, large ones stage and flush their rows
+    // less often; measured with three streams and the launch order by flops
+    // per task, (H2O)32 32.8-32.9 ms at 256, 33.0 at 512, 33.0 at 1024,
+    // 33.5 at 2048 and 35.1 at 4096; C16H34 1.455-1.459 ms at 256,
+    // 1.457-1.458 at 512 and 1.483-1.487 at 4096
+    int group_min_ = 256;
     // lane-per-quartet classes: a task whose lanes would each evaluate more
     // than prim_split_ primitive quartets is split by bra primitive pairs
     // (LIBRINT_PRIM_SPLIT, default 48; 0 = never), so that the heaviest
