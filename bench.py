@@ -39,8 +39,8 @@ HBM_PEAK = 8000.0         # GB/s
 #   COUNTER_FLOPS: executed FP64 VALU flops, 64 x (2 FMA + ADD + MUL + TRANS)
 #            wavefront instructions.
 TRAFFIC = {
-    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): (2 * 1.456e6 + 2.614e6) * 1024,
-    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, false, false>"): (2 * 2.254e4 + 1.486e5) * 1024,
+    ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): (2 * 1.463e6 + 2.614e6) * 1024,
+    ("c16h34_631g", "fp64", "jk_group_kernel<0, 1, 0, 0, false, false>"): (2 * 2.229e4 + 1.484e5) * 1024,
 }
 COUNTER_FLOPS = {
     ("w32_et", "fp64", "jk_group_kernel<0, 1, 0, 0, false, true>"): 6.6876e10,
