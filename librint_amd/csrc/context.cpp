@@ -582,7 +582,9 @@ void Context::build_tasks()
             // density rows in LDS and up to 768-task groups 36.35 -> 36.03 ms)
             const bool cheap = (double)A.stat.prim_quartets < 4.0 * (double)A.stat.quartets;
             cap = std::min(gmax, std::max(8, (rows * nao_ + 63) / 64 * (cheap ? (gdrow ? 12 : 4) : 1)));
-            cap = std::max(8, std::min(cap, (int)(A.tasks.size() / group_min_)));
+            // (per share when the build is dealt over shares: each share's
+            // launch keeps group_min_ groups to balance over the device)
+            cap = std::max(8, std::min(cap, (int)(A.tasks.size() / ((size_t)group_min_ * std::max(1, nshards_)))));
             cap = std::min(cap, gmax);
         }
         for (size_t t = 0; t < A.tasks.size();) {
